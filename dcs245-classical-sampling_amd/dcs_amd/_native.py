@@ -1,0 +1,267 @@
+"""ctypes binding of libdcs_amd.so (C ABI declared in include/dcs_amd.h).
+
+The shared library is built in place next to this file by csrc/build.sh.  There is no fallback:
+importing the engine without the library raises, so a missing build can never be mistaken for a
+result."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / 'libdcs_amd.so'
+
+DCS_OK, DCS_ERR_INVALID, DCS_ERR_RUNTIME, DCS_ERR_NOMEM, DCS_ERR_NONFINITE, DCS_ERR_ZERO = range(6)
+STORAGE_C64, STORAGE_C128 = 0, 1
+MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED = 0, 1, 2
+
+EXPORTED = (
+    'dcs_last_error', 'dcs_get_device_count', 'dcs_get_device_info', 'dcs_context_create',
+    'dcs_context_destroy', 'dcs_synchronize', 'dcs_plan_create', 'dcs_plan_destroy', 'dcs_plan_units',
+    'dcs_plan_schedule', 'dcs_contract', 'dcs_block_zero', 'dcs_block_accumulate', 'dcs_block_transform',
+    'dcs_block_read', 'dcs_block_device_pointer', 'dcs_draw', 'dcs_frontier_read', 'dcs_plan_profile',
+    'dcs_stream_copy')
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [('name', C.c_char * 256), ('arch', C.c_char * 64), ('total_bytes', C.c_uint64),
+                ('free_bytes', C.c_uint64), ('compute_units', C.c_int32), ('runtime_version', C.c_int32),
+                ('driver_version', C.c_int32), ('device', C.c_int32)]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [('storage', C.c_int32), ('maxslots', C.c_int32), ('lanes', C.c_int32), ('nsteps', C.c_int32),
+                ('kind', C.POINTER(C.c_int32)), ('K', C.POINTER(C.c_uint64)), ('Ka', C.POINTER(C.c_uint64)),
+                ('pivot', C.POINTER(C.c_int32)), ('pivot_a', C.POINTER(C.c_int32)),
+                ('beta', C.POINTER(C.c_uint64)), ('in_mask', C.POINTER(C.c_uint64)),
+                ('out_mask', C.POINTER(C.c_uint64)), ('k', C.c_int32), ('terminal_mask', C.c_uint64),
+                ('syndrome_map', C.POINTER(C.c_uint64)), ('phase', C.POINTER(C.c_double)),
+                ('open_unitaries', C.POINTER(C.c_double)), ('blocks', C.c_int32),
+                ('fuse_max_dim', C.c_int32), ('resident_bits', C.c_int32), ('tile_log2', C.c_int32)]
+
+
+class DrawResult(C.Structure):
+    _fields_ = [('index', C.c_int64), ('weight', C.c_double), ('total', C.c_double),
+                ('cdf_lower', C.c_double), ('cdf_upper', C.c_double), ('margin', C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load the library once; raise ImportError if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f'{LIB_PATH} is missing: build it with csrc/build.sh (or __graft_entry__.build())')
+    lib = C.CDLL(str(LIB_PATH))
+    vp, i32, u64, dbl = C.c_void_p, C.c_int32, C.c_uint64, C.c_double
+    P = C.POINTER
+    sig = {
+        'dcs_last_error': (C.c_char_p, []),
+        'dcs_get_device_count': (C.c_int, [P(C.c_int)]),
+        'dcs_get_device_info': (C.c_int, [C.c_int, P(DeviceInfo)]),
+        'dcs_context_create': (C.c_int, [C.c_int, P(vp)]),
+        'dcs_context_destroy': (None, [vp]),
+        'dcs_synchronize': (C.c_int, [vp]),
+        'dcs_plan_create': (C.c_int, [vp, P(PlanDesc), P(vp)]),
+        'dcs_plan_destroy': (None, [vp]),
+        'dcs_plan_units': (C.c_int, [vp, C.c_int, P(C.c_int)]),
+        'dcs_plan_schedule': (C.c_int, [vp, C.c_int, C.c_int, P(i32), P(i32), P(i32), P(i32), P(C.c_int)]),
+        'dcs_contract': (C.c_int, [vp, C.c_int, C.c_int, P(i32), P(dbl), P(i32), P(i32), P(dbl)]),
+        'dcs_block_zero': (C.c_int, [vp, C.c_int]),
+        'dcs_block_accumulate': (C.c_int, [vp, C.c_int, C.c_int, u64, dbl, dbl]),
+        'dcs_block_transform': (C.c_int, [vp, C.c_int]),
+        'dcs_block_read': (C.c_int, [vp, C.c_int, P(dbl)]),
+        'dcs_block_device_pointer': (C.c_int, [vp, C.c_int, P(vp)]),
+        'dcs_draw': (C.c_int, [vp, C.c_int, dbl, P(DrawResult)]),
+        'dcs_frontier_read': (C.c_int, [vp, C.c_int, vp]),
+        'dcs_plan_profile': (C.c_int, [vp, C.c_int, P(dbl), P(C.c_int64)]),
+        'dcs_stream_copy': (C.c_int, [vp, u64, C.c_int, P(dbl)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    """Translate a dcs_status into the reference's exception types."""
+    if rc == DCS_OK:
+        return
+    msg = load().dcs_last_error().decode(errors='replace')
+    if rc == DCS_ERR_INVALID:
+        raise ValueError(msg)
+    if rc == DCS_ERR_NOMEM:
+        raise MemoryError(msg)
+    if rc in (DCS_ERR_NONFINITE, DCS_ERR_ZERO):
+        raise ArithmeticError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def device_count():
+    n = C.c_int(0)
+    check(load().dcs_get_device_count(C.byref(n)))
+    return n.value
+
+
+def device_info(device=0):
+    info = DeviceInfo()
+    check(load().dcs_get_device_info(device, C.byref(info)))
+    return {'device_name': info.name.decode(), 'arch': info.arch.decode(), 'total_bytes': int(info.total_bytes),
+            'free_bytes': int(info.free_bytes), 'compute_units': int(info.compute_units),
+            'hip_runtime_version': int(info.runtime_version), 'hip_driver_version': int(info.driver_version),
+            'device': int(info.device)}
+
+
+class Context:
+    """One device and one HIP stream (default priority)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.dcs_context_create(int(device), C.byref(h)))
+        self.handle = h
+        self.device = int(device)
+
+    def synchronize(self):
+        check(self.lib.dcs_synchronize(self.handle))
+
+    def stream_copy_ms(self, nbytes, reps=5):
+        ms = C.c_double()
+        check(self.lib.dcs_stream_copy(self.handle, C.c_uint64(nbytes), int(reps), C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.handle:
+            self.lib.dcs_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """Device-resident plan: frontier buffer(s), step tables, syndrome map, phase, block accumulators."""
+
+    def __init__(self, ctx, table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, phase,
+                 open_unitaries, blocks=1, fuse_max_dim=6, resident_bits=12, tile_log2=12):
+        self.ctx, self.lib = ctx, ctx.lib
+        n = len(table['kind'])
+        keep = {}
+
+        def arr(name, dt):
+            a = np.ascontiguousarray(table[name], dtype=dt)
+            keep[name] = a
+            return a
+        d = PlanDesc()
+        d.storage, d.maxslots, d.lanes, d.nsteps = int(storage), int(maxslots), int(lanes), n
+        d.kind = ptr(arr('kind', np.int32), C.c_int32)
+        d.K = ptr(arr('K', np.uint64), C.c_uint64)
+        d.Ka = ptr(arr('Ka', np.uint64), C.c_uint64)
+        d.pivot = ptr(arr('pivot', np.int32), C.c_int32)
+        d.pivot_a = ptr(arr('pivot_a', np.int32), C.c_int32)
+        d.beta = ptr(arr('beta', np.uint64), C.c_uint64)
+        d.in_mask = ptr(arr('in_mask', np.uint64), C.c_uint64)
+        d.out_mask = ptr(arr('out_mask', np.uint64), C.c_uint64)
+        smap = np.ascontiguousarray(syndrome_map, dtype=np.uint64)
+        ph = np.ascontiguousarray(phase, dtype=np.float64)
+        U = np.ascontiguousarray(np.asarray(open_unitaries, dtype=np.complex128).reshape(-1, 4).view(np.float64))
+        d.k, d.terminal_mask = int(k), C.c_uint64(int(terminal_mask))
+        d.syndrome_map = ptr(smap, C.c_uint64)
+        d.phase = ptr(ph, C.c_double)
+        d.open_unitaries = ptr(U, C.c_double)
+        d.blocks, d.fuse_max_dim, d.resident_bits, d.tile_log2 = int(blocks), int(fuse_max_dim), int(resident_bits), int(tile_log2)
+        h = C.c_void_p()
+        check(self.lib.dcs_plan_create(ctx.handle, C.byref(d), C.byref(h)))
+        self.handle = h
+        self.nsteps, self.k, self.lanes, self.blocks = n, int(k), int(lanes), int(blocks)
+        self.storage = int(storage)
+        self.terminal_rank = bin(int(terminal_mask)).count('1')
+
+    def units(self, mode):
+        u = C.c_int()
+        check(self.lib.dcs_plan_units(self.handle, int(mode), C.byref(u)))
+        return u.value
+
+    def schedule(self, mode):
+        cap = self.nsteps + 1
+        kinds, first, count, dim = (np.zeros(cap, np.int32) for _ in range(4))
+        nseg = C.c_int()
+        check(self.lib.dcs_plan_schedule(self.handle, int(mode), cap, ptr(kinds, C.c_int32), ptr(first, C.c_int32),
+                                         ptr(count, C.c_int32), ptr(dim, C.c_int32), C.byref(nseg)))
+        m = nseg.value
+        return [(int(kinds[i]), int(first[i]), int(count[i]), int(dim[i])) for i in range(m)]
+
+    def contract(self, mode, lanes, coeffs, ov, y):
+        """Run every step for the given lanes; returns float64[nlanes, nunits] of unit maxima."""
+        lanes = np.ascontiguousarray(lanes, dtype=np.int32)
+        nl = len(lanes)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(nl, self.nsteps, 4)
+        ov = np.ascontiguousarray(ov, dtype=np.int32).reshape(nl, self.nsteps)
+        y = np.ascontiguousarray(y, dtype=np.int32).reshape(nl, self.nsteps)
+        out = np.zeros((nl, self.units(mode)), np.float64)
+        check(self.lib.dcs_contract(self.handle, int(mode), nl, ptr(lanes, C.c_int32), ptr(coeffs, C.c_double),
+                                    ptr(ov, C.c_int32), ptr(y, C.c_int32), ptr(out, C.c_double)))
+        return out
+
+    def block_zero(self, block=0):
+        check(self.lib.dcs_block_zero(self.handle, int(block)))
+
+    def block_accumulate(self, block, lane, xmask, coef):
+        coef = complex(coef)
+        check(self.lib.dcs_block_accumulate(self.handle, int(block), int(lane), C.c_uint64(int(xmask)),
+                                            coef.real, coef.imag))
+
+    def block_transform(self, block=0):
+        check(self.lib.dcs_block_transform(self.handle, int(block)))
+
+    def block_read(self, block=0):
+        out = np.empty(1 << self.k, np.complex128)
+        check(self.lib.dcs_block_read(self.handle, int(block), ptr(out.view(np.float64), C.c_double)))
+        return out
+
+    def block_pointer(self, block=0):
+        p = C.c_void_p()
+        check(self.lib.dcs_block_device_pointer(self.handle, int(block), C.byref(p)))
+        return p.value
+
+    def draw(self, block, u):
+        r = DrawResult()
+        check(self.lib.dcs_draw(self.handle, int(block), float(u), C.byref(r)))
+        return {'A_index': int(r.index), 'selected_scaled_weight': r.weight, 'scaled_mass_cdf': r.total,
+                'cdf_lower_numpy': r.cdf_lower, 'cdf_upper_numpy': r.cdf_upper, 'cdf_margin': r.margin}
+
+    def frontier_terminal(self, lane=0):
+        dt = np.complex64 if self.storage == STORAGE_C64 else np.complex128
+        out = np.empty(1 << self.terminal_rank, dt)
+        check(self.lib.dcs_frontier_read(self.handle, int(lane), out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def profile(self, enable):
+        ms = np.zeros(5, np.float64)
+        n = np.zeros(5, np.int64)
+        check(self.lib.dcs_plan_profile(self.handle, int(bool(enable)), ptr(ms, C.c_double), ptr(n, C.c_int64)))
+        return ms, n
+
+    def close(self):
+        if getattr(self, 'handle', None):
+            self.lib.dcs_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

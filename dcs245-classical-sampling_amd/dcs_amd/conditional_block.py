@@ -1,0 +1,197 @@
+"""Coherent, parity-sliced conditional output blocks on one MI355X.
+
+Drop-in for the reference engine classes (sampling/conditional_block_gpu.py:24-177,
+sampling/conditional_block_inplace_gpu.py:17-38, kernel_opt/inplace_gpu.py:219-263):
+
+    engine = InplaceConditionalBlockGPU(graph, unitaries, n, order, parity_operations, open_vertices)
+    engine.alloc(dtype=np.complex64)            # or np.complex128 for audit blocks
+    vector, meta = engine.block(x_bits)         # vector: DeviceBlock (np.asarray(vector) copies it)
+
+The host keeps the reference's bookkeeping (local vectors, slice coefficients, offsets, the log-scale
+sum); every pass over a frontier, the slice scatter, the Walsh-Hadamard / phase / Clifford tail and
+the CDF draw run in libdcs_amd.so kernels.  There is no CPU fallback: alloc() raises when the library
+or the device is missing.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import _native
+from .block_plan import BlockPlan
+
+SCHEDULES = {'steps': _native.MODE_STEPS, 'steps_host': _native.MODE_STEPS_HOST, 'fused': _native.MODE_FUSED}
+
+
+def logscale_from_maxima(m2):
+    """Replay the reference's host scale chain over the unit maxima of one slice
+    (kernel_opt/inplace_gpu.py:243-258): before unit u, e = frexp(sqrt(previous max))[1] and
+    ls += e ln 2.  Returns (ls, alive); alive is False when a unit's maximum is exactly zero
+    (slice skipped).  A nonfinite maximum raises ArithmeticError."""
+    ls = 0.
+    maximum = 1.
+    for m in m2:
+        e = math.frexp(maximum)[1]
+        ls += e * math.log(2.)
+        m = float(m)
+        if not np.isfinite(m):
+            raise ArithmeticError('Nonfinite in-place environment')
+        if m == 0:
+            return ls, False
+        maximum = math.sqrt(m)
+    return ls, True
+
+
+class DeviceBlock:
+    """The 2^k complex128 block left on the device by block().  np.asarray(), .get() or
+    cupy-style asnumpy-by-copy return a host copy; .ptr is the device address."""
+
+    def __init__(self, engine, index):
+        self._engine = engine
+        self._index = index
+        self.shape = (1 << engine.k,)
+        self.dtype = np.dtype(np.complex128)
+        self.ptr = engine._plan.block_pointer(index)
+
+    def __len__(self):
+        return self.shape[0]
+
+    def get(self):
+        return self._engine._plan.block_read(self._index)
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.get()
+        return a if dtype is None else a.astype(dtype)
+
+    def draw(self, uniform):
+        """Device CDF draw (primary numpy semantics of draw_checked) on this block."""
+        return self._engine._plan.draw(self._index, uniform)
+
+
+class InplaceConditionalBlockGPU:
+    """Reference-compatible engine (conditional_block_inplace_gpu.py:194-215)."""
+
+    def __init__(self, graph, unitaries, n, order, parity_operations, open_vertices):
+        self.bp = bp = BlockPlan(graph, unitaries, n, order, parity_operations, open_vertices)
+        # attributes the reference exposes
+        self.N_full, self.N, self.n = bp.N_full, bp.N, bp.n
+        self.open_full, self.open_red = bp.open_full, bp.open_red
+        self.slices, self.slice_edges = bp.slices, bp.slice_edges
+        self.order, self.fixed, self.fut = bp.order, bp.fixed, bp.fut
+        self.steps, self.stop, self.rank_profile = bp.steps, bp.stop, bp.rank_profile
+        self.rmax, self.work = bp.rmax, bp.work
+        self.terminal_basis, self.terminal_rank = bp.gather_terminal_basis, len(bp.gather_terminal_basis)
+        self.open_edges = bp.open_edges
+        self.U_full = bp.U_full
+        self.k = bp.k
+        self.profile = False
+        self._plan = None
+
+    def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='fused',
+              fuse_max_dim=6, resident_bits=12, tile_log2=12, blocks=1):
+        """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
+
+        dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
+        lanes      frontiers held at once; slices of one block are contracted `lanes` at a time
+        schedule   'fused' (resident runs + coset groups), 'steps' (one launch per step, scale chain
+                   on the device) or 'steps_host' (one launch per step, host reads every maximum)
+        The legacy fp64_kernel_path argument is accepted and ignored, as in the reference."""
+        dt = np.dtype(dtype)
+        if dt not in (np.dtype(np.complex64), np.dtype(np.complex128)):
+            raise ValueError(dtype)
+        if schedule not in SCHEDULES:
+            raise ValueError(f'unknown schedule {schedule!r}')
+        bp = self.bp
+        self.inplace_fp64 = self.use_fp64 = dt == np.dtype(np.complex128)
+        self.inplace_steps, self.inplace_maxslots, self.inplace_rmax = bp.inplace_steps, bp.inplace_maxslots, bp.inplace_rmax
+        self.terminal_basis, self.terminal_rank = bp.terminal_basis, bp.terminal_rank
+        if bp.terminal_rank > bp.k:
+            raise ValueError(f'terminal rank {bp.terminal_rank} exceeds the {bp.k} open vertices')
+        self.mode = SCHEDULES[schedule]
+        self.lanes = int(lanes)
+        self.ctx = _native.Context(device)
+        storage = _native.STORAGE_C128 if self.use_fp64 else _native.STORAGE_C64
+        self._plan = _native.Plan(self.ctx, bp.step_table(), bp.inplace_maxslots, storage, self.lanes, bp.k,
+                                  bp.terminal_mask(), bp.syndrome_map(), bp.phase(), bp.open_unitaries(),
+                                  blocks=blocks, fuse_max_dim=fuse_max_dim, resident_bits=resident_bits,
+                                  tile_log2=tile_log2)
+        self.buf = True
+
+    # ---- reference mixin hooks -------------------------------------------------------------
+    def prefix_inplace(self, f, yfix, lane=0):
+        """(env or None, logscale, offset) of one slice (kernel_opt/inplace_gpu.py:242-259).
+        env is the host copy of the terminal environment in storage precision."""
+        coeffs, ov, ys, offset = self.bp.step_parameters(f, yfix)
+        m2 = self._plan.contract(self.mode, [lane], coeffs, ov, ys)[0]
+        ls, alive = logscale_from_maxima(m2)
+        if not alive:
+            return None, ls, offset
+        return self._plan.frontier_terminal(lane), ls, offset
+
+    def _prefix(self, f, yfix):
+        env, ls, offset = self.prefix_inplace(f, yfix)
+        return env, ls, offset >> self.stop
+
+    def _sweep(self, f, yfix, renorm_every=None):
+        env, ls, _ = self.prefix_inplace(f, yfix)
+        return (0j if env is None else complex(env[0])), ls
+
+    # ---- the block ---------------------------------------------------------------------------
+    def _slice_jobs(self, x_bits, w, part, parts):
+        f_full = self.bp.local_vectors(x_bits, w=w)
+        jobs, count = [], 0
+        for count, coef, f, yfix, freeflip in self.bp.slice_assignments(f_full, part, parts):
+            if f is None:
+                continue
+            coeffs, ov, ys, offset = self.bp.step_parameters(f, yfix)
+            jobs.append((coef, coeffs, ov, ys, offset, freeflip))
+        return jobs, count
+
+    def block(self, x_bits, w=None, part=0, parts=1, block_index=0):
+        """Scaled conditional amplitudes sqrt(D) psi[prefix, :] (conditional_block_gpu.py:128-177).
+
+        Slices are contracted `lanes` at a time; each one's environment is scattered into the
+        complex128 accumulator with its coefficient exp(ls - (N_full - n + k) ln2 / 2), then the
+        tail is applied once."""
+        if self._plan is None:
+            raise RuntimeError('call alloc() first')
+        start = time.perf_counter()
+        plan = self._plan
+        jobs, count = self._slice_jobs(x_bits, w, part, parts)
+        plan.block_zero(block_index)
+        for i in range(0, len(jobs), self.lanes):
+            batch = jobs[i:i + self.lanes]
+            lanes = np.arange(len(batch), dtype=np.int32)
+            m2 = plan.contract(self.mode, lanes, np.stack([j[1] for j in batch]), np.stack([j[2] for j in batch]),
+                               np.stack([j[3] for j in batch]))
+            for lane, (coef, _, _, _, offset, freeflip) in enumerate(batch):
+                ls, alive = logscale_from_maxima(m2[lane])
+                if not alive:
+                    continue
+                c = coef * self.bp.slice_scale(ls)
+                plan.block_accumulate(block_index, lane, (offset >> self.stop) ^ freeflip, c)
+        plan.block_transform(block_index)
+        self.ctx.synchronize()
+        vec = DeviceBlock(self, block_index)
+        return vec, {'seconds': time.perf_counter() - start, 'part': part, 'parts': parts,
+                     'slices_evaluated': count,
+                     'prefix_dtype': 'complex128' if self.use_fp64 else 'complex64',
+                     'rmax': self.rmax, 'terminal_rank': self.terminal_rank,
+                     'work_per_slice': self.work, 'open_vertices': self.open_full,
+                     'vector_scale': 'sqrt(2^(n-k)); sum all complex parts before probabilities'}
+
+    def schedule(self):
+        return self._plan.schedule(self.mode)
+
+    def close(self):
+        if self._plan is not None:
+            self._plan.close()
+            self._plan = None
+        if getattr(self, 'ctx', None) is not None:
+            self.ctx.close()
+            self.ctx = None
+
+
+ConditionalBlockGPU = InplaceConditionalBlockGPU

@@ -1,0 +1,1415 @@
+// libdcs_amd: MI355X (gfx950) executor of the preserved-coordinate frontier contraction, the
+// coherent conditional block and its CDF draw.  C ABI in include/dcs_amd.h.
+//
+// Kernels
+//   k_step      one in-place step over the whole frontier, one thread per orbit; fused
+//               power-of-two scale and max |out|^2 (kernel_opt/inplace_gpu.py:155-198).
+//   k_resident  a run of consecutive narrow steps whose live slots all sit below 2^RB: the frontier
+//               stays in LDS, one workgroup per lane, the exact per-step scale chain.
+//   k_coset     a fused group of consecutive steps.  The XOR masks of the group span a subspace V
+//               (dim <= 8) of the address space; every coset c0 + V is closed under all steps of
+//               the group, so a workgroup loads a tile of cosets from HBM once, applies every step
+//               in LDS and writes it back once.  Values are rounded to storage precision after
+//               each step exactly as in k_step; only the power-of-two scale is applied once per
+//               group, so the stored mantissas equal those of the per-step schedule.
+//   k_scatter, k_wht_pass, k_phase, k_u_pass
+//               coherent slice sum and the Walsh-Hadamard / phase / local-Clifford tail
+//               (sampling/conditional_block_gpu.py:155-171).
+//   k_cumsum, k_search
+//               |a|^2, the sequential float64 cumsum, normalisation and searchsorted(side='right')
+//               of draw_checked (sampling/sampling_protocol.py:56-65); the single-wave sequential
+//               scan reproduces numpy's accumulation order bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/dcs_amd.h"
+#include "dcs_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(DCS_ERR_RUNTIME, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                  \
+    } while (0)
+
+constexpr int kThreads = 256;
+constexpr int kMaxRuns = 16;
+constexpr int kMaxDim = 8;
+
+struct Runs {
+    int n;  // number of runs; -1: use pdep over `mask`
+    uint8_t src[kMaxRuns], len[kMaxRuns], dst[kMaxRuns];
+    u64 mask;
+};
+
+Runs make_runs(u64 mask) {
+    Runs r;
+    memset(&r, 0, sizeof(r));
+    r.mask = mask;
+    int src = 0, n = 0;
+    for (int p = 0; p < 64;) {
+        if (!((mask >> p) & 1)) {
+            ++p;
+            continue;
+        }
+        int q = p;
+        while (q < 64 && ((mask >> q) & 1)) ++q;
+        if (n == kMaxRuns) {
+            r.n = -1;
+            return r;
+        }
+        r.src[n] = (uint8_t)src;
+        r.len[n] = (uint8_t)(q - p);
+        r.dst[n] = (uint8_t)p;
+        src += q - p;
+        ++n;
+        p = q;
+    }
+    r.n = n;
+    return r;
+}
+
+__device__ __forceinline__ u64 deposit(u64 x, const Runs &r) {
+    if (r.n < 0) return pdep_loop(x, r.mask);
+    u64 c = 0;
+    for (int i = 0; i < r.n; ++i) c |= ((x >> r.src[i]) & ((1ull << r.len[i]) - 1)) << r.dst[i];
+    return c;
+}
+
+struct Dyn {
+    double f0r, f0i, f1r, f1i;
+    int32_t ov, y;
+    int32_t pad[2];
+};
+
+struct StepArgs {
+    u64 norbits, K, Ka, beta;
+    int32_t kind, pivot;
+    Runs runs;
+};
+
+struct StepDev {
+    u64 K, Ka, beta, slots, in_mask, out_mask;
+    int32_t kind, pivot, pivot_a, nslots;
+};
+
+struct GroupDev {
+    int32_t d, wbits, lane_identity, nsteps;
+    int32_t first_gstep, addrv_offset;
+    uint32_t dead_in_t, dead_out_t;
+    u64 rep_low_mask;
+    u64 ntiles;
+    Runs high;
+};
+
+struct GStepDev {
+    u64 beta;
+    int32_t kind, step, tp, tpa;
+    uint32_t tk, tka, bvec, beta_lane, inactive_t, pad;
+};
+
+struct DrawOut {
+    int64_t index;
+    double weight, total, lower, upper, margin;
+    int32_t status;
+    int32_t pad;
+};
+
+__device__ __forceinline__ double bits2d(u64 b) { return __longlong_as_double((long long)b); }
+
+// Wave-level max, then one atomic per wave (values are >= 0 or NaN, so the u64 order of the
+// float64 bit patterns is their numeric order, NaN above +inf).
+__device__ __forceinline__ void commit_max(double lm, u64 *slot) {
+    for (int off = 32; off > 0; off >>= 1) lm = max_nan(lm, __shfl_xor(lm, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(slot, (u64)__double_as_longlong(lm));
+}
+
+template <class S>
+__device__ __forceinline__ void put_scaled(S *E, u64 c, cd v, double scale, double &lm) {
+    const S o = Store<S>::round(cd{v.r * scale, v.i * scale});
+    E[c] = o;
+    lm = max_nan(lm, Store<S>::norm2(o));
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_step: one step, one thread per orbit, orbit index deposited into the enumerated slots.
+template <class S, int KIND>
+__global__ void __launch_bounds__(kThreads) k_step(S *__restrict__ base, u64 lane_stride,
+                                                   const int *__restrict__ lane_ids, StepArgs a,
+                                                   const Dyn *__restrict__ dyn, int nsteps, int step,
+                                                   u64 *__restrict__ dmax, int nunits, int unit) {
+    const int li = blockIdx.y;
+    S *__restrict__ E = base + (u64)lane_ids[li] * lane_stride;
+    const Dyn dd = dyn[(u64)li * nsteps + step];
+    const double prev = unit ? bits2d(dmax[(u64)li * nunits + unit - 1]) : 1.0;
+    const double scale = scale_from_m2(prev);
+    const cd f0{dd.f0r, dd.f0i}, f1{dd.f1r, dd.f1i};
+    const int ov = dd.ov, y = dd.y;
+    const u64 beta = a.beta;
+    double lm = 0.0;
+    const u64 stride = (u64)blockDim.x * gridDim.x;
+    for (u64 x = (u64)blockIdx.x * blockDim.x + threadIdx.x; x < a.norbits; x += stride) {
+        const u64 c = deposit(x, a.runs);
+        const cd va = Store<S>::load(E[c]);
+        const int sc = sign_bit(c, beta, ov);
+        if (KIND == 0) {
+            cd o0, o1;
+            orb_grow(va, f0, f1, sc, o0, o1);
+            put_scaled(E, c, o0, scale, lm);
+            put_scaled(E, c | (1ull << a.pivot), o1, scale, lm);
+        } else if (KIND == 1) {
+            const u64 d = c ^ a.Ka;
+            const cd vb = Store<S>::load(E[d]);
+            cd o0, o1;
+            orb_mix_pair(va, vb, f0, f1, sc, sign_bit(d, beta, ov), o0, o1);
+            put_scaled(E, c, o0, scale, lm);
+            put_scaled(E, d, o1, scale, lm);
+        } else if (KIND == 2) {
+            put_scaled(E, c, orb_mix_diag(va, f0, f1, sc), scale, lm);
+        } else if (KIND == 3) {
+            const u64 d = c ^ a.K;
+            const cd vb = Store<S>::load(E[d]);
+            cd o0, o1;
+            orb_replace(va, vb, f0, f1, sc, sign_bit(d, beta, ov), o0, o1);
+            put_scaled(E, c, o0, scale, lm);
+            put_scaled(E, d, o1, scale, lm);
+        } else if (KIND == 4) {
+            put_scaled(E, c, orb_fixed(va, f0, f1, sc, y), scale, lm);
+        } else if (KIND == 5) {
+            const u64 d = c ^ a.K;
+            const cd vb = Store<S>::load(E[d]);
+            put_scaled(E, c, orb_fixed_shrink(va, vb, f0, f1, sc, sign_bit(d, beta, ov), y), scale, lm);
+        } else if (KIND == 6) {
+            const u64 c1 = c ^ a.K, d = c ^ a.Ka, d1 = d ^ a.K;
+            const cd vb = Store<S>::load(E[c1]), ve = Store<S>::load(E[d]), vg = Store<S>::load(E[d1]);
+            cd o0, o1;
+            orb_shrink_quad(va, vb, ve, vg, f0, f1, sc, sign_bit(c1, beta, ov), sign_bit(d, beta, ov),
+                            sign_bit(d1, beta, ov), o0, o1);
+            put_scaled(E, c, o0, scale, lm);
+            put_scaled(E, d, o1, scale, lm);
+        } else {
+            const u64 d = c ^ a.K;
+            const cd vb = Store<S>::load(E[d]);
+            put_scaled(E, c, orb_shrink_pair(va, vb, f0, f1, sc, sign_bit(d, beta, ov)), scale, lm);
+        }
+    }
+    commit_max(lm, &dmax[(u64)li * nunits + unit]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Apply one step of kind `kind` to a frontier held in LDS (physical address == LDS index).
+template <class S>
+__device__ __forceinline__ void resident_step(S *tile, const StepDev &st, const Dyn &dd, double scale,
+                                              double &lm) {
+    const cd f0{dd.f0r, dd.f0i}, f1{dd.f1r, dd.f1i};
+    const int ov = dd.ov, y = dd.y;
+    const u64 beta = st.beta;
+    const u64 norb = 1ull << st.nslots;
+    for (u64 x = threadIdx.x; x < norb; x += blockDim.x) {
+        const u64 c = pdep_loop(x, st.slots);
+        const cd va = Store<S>::load(tile[c]);
+        const int sc = sign_bit(c, beta, ov);
+        switch (st.kind) {
+            case 0: {
+                cd o0, o1;
+                orb_grow(va, f0, f1, sc, o0, o1);
+                put_scaled(tile, c, o0, scale, lm);
+                put_scaled(tile, c | (1ull << st.pivot), o1, scale, lm);
+            } break;
+            case 1: {
+                const u64 d = c ^ st.Ka;
+                const cd vb = Store<S>::load(tile[d]);
+                cd o0, o1;
+                orb_mix_pair(va, vb, f0, f1, sc, sign_bit(d, beta, ov), o0, o1);
+                put_scaled(tile, c, o0, scale, lm);
+                put_scaled(tile, d, o1, scale, lm);
+            } break;
+            case 2:
+                put_scaled(tile, c, orb_mix_diag(va, f0, f1, sc), scale, lm);
+                break;
+            case 3: {
+                const u64 d = c ^ st.K;
+                const cd vb = Store<S>::load(tile[d]);
+                cd o0, o1;
+                orb_replace(va, vb, f0, f1, sc, sign_bit(d, beta, ov), o0, o1);
+                put_scaled(tile, c, o0, scale, lm);
+                put_scaled(tile, d, o1, scale, lm);
+            } break;
+            case 4:
+                put_scaled(tile, c, orb_fixed(va, f0, f1, sc, y), scale, lm);
+                break;
+            case 5: {
+                const u64 d = c ^ st.K;
+                const cd vb = Store<S>::load(tile[d]);
+                put_scaled(tile, c, orb_fixed_shrink(va, vb, f0, f1, sc, sign_bit(d, beta, ov), y), scale, lm);
+            } break;
+            case 6: {
+                const u64 c1 = c ^ st.K, d = c ^ st.Ka, d1 = d ^ st.K;
+                const cd vb = Store<S>::load(tile[c1]), ve = Store<S>::load(tile[d]),
+                         vg = Store<S>::load(tile[d1]);
+                cd o0, o1;
+                orb_shrink_quad(va, vb, ve, vg, f0, f1, sc, sign_bit(c1, beta, ov), sign_bit(d, beta, ov),
+                                sign_bit(d1, beta, ov), o0, o1);
+                put_scaled(tile, c, o0, scale, lm);
+                put_scaled(tile, d, o1, scale, lm);
+            } break;
+            default: {
+                const u64 d = c ^ st.K;
+                const cd vb = Store<S>::load(tile[d]);
+                put_scaled(tile, c, orb_shrink_pair(va, vb, f0, f1, sc, sign_bit(d, beta, ov)), scale, lm);
+            } break;
+        }
+    }
+}
+
+__device__ __forceinline__ double block_max(double lm, double *red) {
+    for (int off = 32; off > 0; off >>= 1) lm = max_nan(lm, __shfl_xor(lm, off));
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[wave] = lm;
+    __syncthreads();
+    double m = red[0];
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 1; w < nw; ++w) m = max_nan(m, red[w]);
+    __syncthreads();
+    return m;
+}
+
+template <class S, int RB>
+__global__ void __launch_bounds__(kThreads) k_resident(S *__restrict__ base, u64 lane_stride,
+                                                       const int *__restrict__ lane_ids,
+                                                       const StepDev *__restrict__ steps, int s0, int s1,
+                                                       const Dyn *__restrict__ dyn, int nsteps,
+                                                       u64 *__restrict__ dmax, int nunits, int unit0) {
+    __shared__ S tile[1 << RB];
+    __shared__ double red[kThreads / 64];
+    const int li = blockIdx.x;
+    S *__restrict__ E = base + (u64)lane_ids[li] * lane_stride;
+    const u64 live_in = steps[s0].in_mask;
+    for (u64 e = threadIdx.x; e < (1ull << RB); e += blockDim.x)
+        tile[e] = (e & ~live_in) ? Store<S>::zero() : E[e];
+    double prev = unit0 ? bits2d(dmax[(u64)li * nunits + unit0 - 1]) : 1.0;
+    __syncthreads();
+    for (int s = s0; s < s1; ++s) {
+        const StepDev st = steps[s];
+        const Dyn dd = dyn[(u64)li * nsteps + s];
+        double lm = 0.0;
+        resident_step(tile, st, dd, scale_from_m2(prev), lm);
+        const double m2 = block_max(lm, red);  // contains the barrier that orders this step's writes
+        if (threadIdx.x == 0) dmax[(u64)li * nunits + unit0 + (s - s0)] = (u64)__double_as_longlong(m2);
+        prev = m2;
+    }
+    const u64 live_out = steps[s1 - 1].out_mask;
+    for (u64 e = threadIdx.x; e < (1ull << RB); e += blockDim.x)
+        if (!(e & ~live_out)) E[e] = tile[e];
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_coset: fused group.  Tile layout: element (t, l) at LDS index t*W + l, t in [0, 2^d) the
+// coordinate in the RREF basis of V (address = c0(l) ^ addrv[t]), l the lane coordinate.
+template <class S>
+__device__ __forceinline__ void coset_step(S *tile, const GStepDev &st, const Dyn &dd, int d, int wb, int ptile) {
+    const cd f0{dd.f0r, dd.f0i}, f1{dd.f1r, dd.f1i};
+    const int y = dd.y;
+    const uint32_t W = 1u << wb;
+    const int kind = st.kind;
+    const int nfix = (kind == 2 || kind == 4) ? 0 : (kind == 6 ? 2 : 1);
+    const uint32_t norb = (1u << (d - nfix)) << wb;
+    const int lo = min(st.tp, st.tpa), hi = max(st.tp, st.tpa);
+    for (uint32_t o = threadIdx.x; o < norb; o += blockDim.x) {
+        const uint32_t l = o & (W - 1);
+        const uint32_t tt = o >> wb;
+        uint32_t t;
+        if (nfix == 0) t = tt;
+        else if (nfix == 1) t = insert0(tt, st.tp);
+        else t = insert0(insert0(tt, lo), hi);
+        if (t & st.inactive_t) continue;
+        const int pl = ptile ^ (__popc(l & st.beta_lane) & 1);
+#define SB(tv) (pl ^ (__popc((tv) & st.bvec) & 1))
+        const uint32_t ic = t * W + l;
+        const cd va = Store<S>::load(tile[ic]);
+        const int sc = SB(t);
+        switch (kind) {
+            case 0: {
+                cd o0, o1;
+                orb_grow(va, f0, f1, sc, o0, o1);
+                tile[ic] = Store<S>::round(o0);
+                tile[(t | (1u << st.tp)) * W + l] = Store<S>::round(o1);
+            } break;
+            case 1: {
+                const uint32_t u = t ^ st.tka;
+                const uint32_t iu = u * W + l;
+                const cd vb = Store<S>::load(tile[iu]);
+                cd o0, o1;
+                orb_mix_pair(va, vb, f0, f1, sc, SB(u), o0, o1);
+                tile[ic] = Store<S>::round(o0);
+                tile[iu] = Store<S>::round(o1);
+            } break;
+            case 2:
+                tile[ic] = Store<S>::round(orb_mix_diag(va, f0, f1, sc));
+                break;
+            case 3: {
+                const uint32_t u = t ^ st.tk;
+                const uint32_t iu = u * W + l;
+                const cd vb = Store<S>::load(tile[iu]);
+                cd o0, o1;
+                orb_replace(va, vb, f0, f1, sc, SB(u), o0, o1);
+                tile[ic] = Store<S>::round(o0);
+                tile[iu] = Store<S>::round(o1);
+            } break;
+            case 4:
+                tile[ic] = Store<S>::round(orb_fixed(va, f0, f1, sc, y));
+                break;
+            case 5: {
+                const uint32_t u = t ^ st.tk;
+                const cd vb = Store<S>::load(tile[u * W + l]);
+                tile[ic] = Store<S>::round(orb_fixed_shrink(va, vb, f0, f1, sc, SB(u), y));
+            } break;
+            case 6: {
+                const uint32_t c1 = t ^ st.tk, dd_ = t ^ st.tka, d1 = dd_ ^ st.tk;
+                const cd vb = Store<S>::load(tile[c1 * W + l]);
+                const cd ve = Store<S>::load(tile[dd_ * W + l]);
+                const cd vg = Store<S>::load(tile[d1 * W + l]);
+                cd o0, o1;
+                orb_shrink_quad(va, vb, ve, vg, f0, f1, sc, SB(c1), SB(dd_), SB(d1), o0, o1);
+                tile[ic] = Store<S>::round(o0);
+                tile[dd_ * W + l] = Store<S>::round(o1);
+            } break;
+            default: {
+                const uint32_t u = t ^ st.tk;
+                const cd vb = Store<S>::load(tile[u * W + l]);
+                tile[ic] = Store<S>::round(orb_shrink_pair(va, vb, f0, f1, sc, SB(u)));
+            } break;
+        }
+#undef SB
+    }
+}
+
+template <class S, int TL>
+__global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 lane_stride,
+                                                    const int *__restrict__ lane_ids,
+                                                    const GroupDev *__restrict__ groups, int gi,
+                                                    const GStepDev *__restrict__ gsteps,
+                                                    const u64 *__restrict__ addrv_all,
+                                                    const Dyn *__restrict__ dyn, int nsteps,
+                                                    u64 *__restrict__ dmax, int nunits, int unit) {
+    __shared__ S tile[1 << TL];
+    __shared__ u64 addrv[1 << kMaxDim];
+    const GroupDev &g = groups[gi];
+    const int li = blockIdx.y;
+    S *__restrict__ E = base + (u64)lane_ids[li] * lane_stride;
+    const int d = g.d, wb = g.wbits;
+    const uint32_t W = 1u << wb;
+    const uint32_t n = (1u << d) << wb;
+    for (uint32_t t = threadIdx.x; t < (1u << d); t += blockDim.x) addrv[t] = addrv_all[g.addrv_offset + t];
+    const double prev = unit ? bits2d(dmax[(u64)li * nunits + unit - 1]) : 1.0;
+    const double scale = scale_from_m2(prev);
+    double lm = 0.0;
+    __syncthreads();
+    for (u64 ti = blockIdx.x; ti < g.ntiles; ti += gridDim.x) {
+        const u64 tb = deposit(ti, g.high);
+        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+            const uint32_t t = e >> wb, l = e & (W - 1);
+            S v = Store<S>::zero();
+            if (!(t & g.dead_in_t)) {
+                const u64 lp = g.lane_identity ? (u64)l : pdep_loop(l, g.rep_low_mask);
+                v = E[tb ^ lp ^ addrv[t]];
+            }
+            tile[e] = v;
+        }
+        __syncthreads();
+        for (int s = 0; s < g.nsteps; ++s) {
+            const GStepDev st = gsteps[g.first_gstep + s];
+            const Dyn dd = dyn[(u64)li * nsteps + st.step];
+            const int ptile = (__popcll(tb & st.beta) & 1) ^ dd.ov;
+            coset_step(tile, st, dd, d, wb, ptile);
+            __syncthreads();
+        }
+        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+            const uint32_t t = e >> wb, l = e & (W - 1);
+            if (t & g.dead_out_t) continue;
+            const u64 lp = g.lane_identity ? (u64)l : pdep_loop(l, g.rep_low_mask);
+            put_scaled(E, tb ^ lp ^ addrv[t], Store<S>::load(tile[e]), scale, lm);
+        }
+        __syncthreads();
+    }
+    commit_max(lm, &dmax[(u64)li * nunits + unit]);
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ void k_fill_u64(u64 *p, u64 n, u64 v) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x) p[i] = v;
+}
+
+template <class S>
+__global__ void k_init_lanes(S *base, u64 lane_stride, const int *lane_ids, int nlanes) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nlanes) {
+        S one = Store<S>::zero();
+        one.x = 1;
+        base[(u64)lane_ids[i] * lane_stride] = one;
+    }
+}
+
+template <class S>
+__global__ void k_scatter(const S *__restrict__ E, Runs term, u64 n, const u64 *__restrict__ smap, u64 xmask,
+                          cd coef, double2 *__restrict__ total) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x) {
+        const cd p = cmul(Store<S>::load(E[deposit(i, term)]), coef);
+        const u64 j = smap[i] ^ xmask;
+        const double2 t = total[j];
+        total[j] = make_double2(t.x + p.r, t.y + p.i);
+    }
+}
+
+template <class S>
+__global__ void k_gather_terminal(const S *__restrict__ E, Runs term, u64 n, S *__restrict__ out) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x)
+        out[i] = E[deposit(i, term)];
+}
+
+__global__ void k_wht_pass(double2 *v, u64 half, int q) {
+    for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < half; p += (u64)blockDim.x * gridDim.x) {
+        const u64 i0 = insert0_64(p, q), i1 = i0 | (1ull << q);
+        const double2 lo = v[i0], hi = v[i1];
+        v[i0] = make_double2(lo.x + hi.x, lo.y + hi.y);
+        v[i1] = make_double2(lo.x - hi.x, lo.y - hi.y);
+    }
+}
+
+__global__ void k_phase(double2 *v, const double *phase, u64 n) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x) {
+        const cd r = cmul(cd{v[i].x, v[i].y}, cd{phase[i], 0.0});
+        v[i] = make_double2(r.r, r.i);
+    }
+}
+
+__global__ void k_u_pass(double2 *v, u64 half, int q, cd u00, cd u01, cd u10, cd u11) {
+    for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < half; p += (u64)blockDim.x * gridDim.x) {
+        const u64 i0 = insert0_64(p, q), i1 = i0 | (1ull << q);
+        const cd lo{v[i0].x, v[i0].y}, hi{v[i1].x, v[i1].y};
+        const cd a = cadd(cmul(u00, lo), cmul(u01, hi));
+        const cd b = cadd(cmul(u10, lo), cmul(u11, hi));
+        v[i0] = make_double2(a.r, a.i);
+        v[i1] = make_double2(b.r, b.i);
+    }
+}
+
+// One wavefront: weights w = re*re + im*im and their strictly sequential float64 running sum.
+__global__ void __launch_bounds__(64) k_cumsum(const double2 *__restrict__ v, u64 n, double *__restrict__ cum) {
+    const int lane = threadIdx.x;
+    double acc = 0.0;
+    for (u64 base = 0; base < n; base += 64) {
+        const u64 i = base + lane;
+        double w = 0.0;
+        if (i < n) {
+            const double2 a = v[i];
+            w = a.x * a.x + a.y * a.y;
+        }
+        double mine = 0.0;
+#pragma unroll
+        for (int t = 0; t < 64; ++t) {
+            acc = acc + __shfl(w, t);
+            if (lane == t) mine = acc;
+        }
+        if (i < n) cum[i] = mine;
+    }
+}
+
+__global__ void k_search(const double2 *__restrict__ v, const double *__restrict__ cum, u64 n, double u,
+                         DrawOut *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    DrawOut r;
+    memset(&r, 0, sizeof(r));
+    const double total = cum[n - 1];
+    r.total = total;
+    if (!(total > 0.0) || !isfinite(total)) {
+        r.status = isfinite(total) ? DCS_ERR_ZERO : DCS_ERR_NONFINITE;
+        *out = r;
+        return;
+    }
+    // normalised cdf: cum[i] / total, last entry forced to 1
+    u64 lo = 0, hi = n;
+    while (lo < hi) {
+        const u64 mid = (lo + hi) >> 1;
+        const double c = (mid == n - 1) ? 1.0 : cum[mid] / total;
+        if (c <= u) lo = mid + 1;
+        else hi = mid;
+    }
+    r.index = (int64_t)lo;
+    if (lo >= n) {
+        r.status = DCS_ERR_ZERO;
+        *out = r;
+        return;
+    }
+    const double2 a = v[lo];
+    r.weight = a.x * a.x + a.y * a.y;
+    r.lower = lo ? ((lo - 1 == n - 1) ? 1.0 : cum[lo - 1] / total) : 0.0;
+    r.upper = (lo == n - 1) ? 1.0 : cum[lo] / total;
+    r.margin = fmin(u - r.lower, r.upper - u);
+    r.status = (r.weight > 0.0) ? DCS_OK : DCS_ERR_ZERO;
+    *out = r;
+}
+
+__global__ void k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, u64 n) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x) dst[i] = src[i];
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host runtime.
+
+struct Seg {
+    int type;  // 0 step, 1 resident run, 2 coset group
+    int first, count, dim, group, unit;
+};
+
+struct StepHost {
+    int kind, pivot, pivot_a;
+    u64 K, Ka, beta, in_mask, out_mask, slots;
+};
+
+inline int top_bit(u64 x) { return 63 - __builtin_clzll(x); }
+inline int popc(u64 x) { return __builtin_popcountll(x); }
+
+}  // namespace
+
+struct dcs_context {
+    int device;
+    hipStream_t stream;
+    int cus;
+};
+
+struct PendingEvent {
+    int family;
+    hipEvent_t a, b;
+};
+
+struct dcs_plan {
+    dcs_context *ctx;
+    int storage, maxslots, lanes, nsteps, k, blocks;
+    int fuse_max_dim, resident_bits, tile_log2;
+    size_t esz;
+    u64 lane_stride;
+    u64 terminal_mask;
+    std::vector<StepHost> steps;
+    std::vector<StepArgs> step_args;
+    std::vector<double> unitaries;  // k*8
+    std::vector<Seg> sched_steps, sched_fused;
+    std::vector<u64> group_ntiles;
+    int units_steps, units_fused;
+    // device memory
+    void *frontier = nullptr;
+    Dyn *d_dyn = nullptr;
+    int *d_lanes = nullptr;
+    u64 *d_dmax = nullptr;
+    StepDev *d_steps = nullptr;
+    GroupDev *d_groups = nullptr;
+    GStepDev *d_gsteps = nullptr;
+    u64 *d_addrv = nullptr;
+    u64 *d_smap = nullptr;
+    double *d_phase = nullptr;
+    double2 *d_blocks = nullptr;
+    double *d_cum = nullptr;
+    void *d_scratch = nullptr;
+    DrawOut *d_draw = nullptr;
+    // pinned staging
+    Dyn *h_dyn = nullptr;
+    int *h_lanes = nullptr;
+    u64 *h_dmax = nullptr;
+    DrawOut *h_draw = nullptr;
+    // profiling
+    bool profiling = false;
+    double prof_ms[5] = {0, 0, 0, 0, 0};
+    int64_t prof_n[5] = {0, 0, 0, 0, 0};
+    std::vector<PendingEvent> pending;
+};
+
+namespace {
+
+int ev_begin(dcs_plan *p, int family, PendingEvent &pe) {
+    if (!p->profiling) return DCS_OK;
+    pe.family = family;
+    HIPCHK(hipEventCreate(&pe.a));
+    HIPCHK(hipEventCreate(&pe.b));
+    HIPCHK(hipEventRecord(pe.a, p->ctx->stream));
+    return DCS_OK;
+}
+int ev_end(dcs_plan *p, PendingEvent &pe) {
+    if (!p->profiling) return DCS_OK;
+    HIPCHK(hipEventRecord(pe.b, p->ctx->stream));
+    p->pending.push_back(pe);
+    return DCS_OK;
+}
+int ev_resolve(dcs_plan *p) {
+    for (auto &pe : p->pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventSynchronize(pe.b));
+        HIPCHK(hipEventElapsedTime(&ms, pe.a, pe.b));
+        p->prof_ms[pe.family] += ms;
+        p->prof_n[pe.family] += 1;
+        (void)hipEventDestroy(pe.a);
+        (void)hipEventDestroy(pe.b);
+    }
+    p->pending.clear();
+    return DCS_OK;
+}
+
+#define PROF(fam, launch)                              \
+    do {                                               \
+        PendingEvent pe_;                              \
+        int rc_ = ev_begin(p, fam, pe_);               \
+        if (rc_) return rc_;                           \
+        launch;                                        \
+        HIPCHK(hipGetLastError());                     \
+        rc_ = ev_end(p, pe_);                          \
+        if (rc_) return rc_;                           \
+    } while (0)
+
+unsigned grid_for(u64 work, int cus, unsigned per_cu) {
+    const u64 blocks = (work + kThreads - 1) / kThreads;
+    const u64 cap = (u64)cus * per_cu;
+    return (unsigned)std::max<u64>(1, std::min(blocks, cap));
+}
+
+u64 orbit_slots(const StepHost &s) {
+    u64 m = s.in_mask;
+    if (s.kind == 1) m &= ~(1ull << s.pivot_a);
+    else if (s.kind == 3 || s.kind == 5 || s.kind == 7) m &= ~(1ull << s.pivot);
+    else if (s.kind == 6) m &= ~((1ull << s.pivot) | (1ull << s.pivot_a));
+    return m;
+}
+
+std::vector<u64> step_masks(const StepHost &s) {
+    switch (s.kind) {
+        case 0: return {1ull << s.pivot};
+        case 1: return {s.Ka};
+        case 3: case 5: case 7: return {s.K};
+        case 6: return {s.K, s.Ka};
+        default: return {};
+    }
+}
+
+// Insert v into an RREF basis keyed by highest bit (pivots distinct, every other vector zero there).
+bool rref_insert(std::vector<u64> &basis, u64 v) {
+    for (u64 b : basis)
+        if ((v >> top_bit(b)) & 1) v ^= b;
+    if (!v) return false;
+    const int p = top_bit(v);
+    for (u64 &b : basis)
+        if ((b >> p) & 1) b ^= v;
+    basis.push_back(v);
+    std::sort(basis.begin(), basis.end(), [](u64 a, u64 b) { return top_bit(a) < top_bit(b); });
+    return true;
+}
+
+bool step_is_resident(const StepHost &s, int rb) {
+    if (rb <= 0) return false;
+    const u64 lim = (rb >= 64) ? ~0ull : ((1ull << rb) - 1);
+    return !(s.in_mask & ~lim) && !(s.out_mask & ~lim);
+}
+
+int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<GStepDev> &gsteps,
+                    std::vector<u64> &addrv) {
+    const int n = p->nsteps;
+    p->sched_steps.clear();
+    for (int s = 0; s < n; ++s) p->sched_steps.push_back(Seg{0, s, 1, 0, -1, s});
+    p->units_steps = n;
+
+    p->sched_fused.clear();
+    int unit = 0;
+    const int D = std::max(1, std::min(kMaxDim, p->fuse_max_dim));
+    const int TL = p->tile_log2;
+    int s = 0;
+    while (s < n) {
+        if (step_is_resident(p->steps[s], p->resident_bits)) {
+            int e = s;
+            while (e < n && step_is_resident(p->steps[e], p->resident_bits)) ++e;
+            p->sched_fused.push_back(Seg{1, s, e - s, 0, -1, unit});
+            unit += e - s;
+            s = e;
+            continue;
+        }
+        if (p->fuse_max_dim <= 0) {
+            p->sched_fused.push_back(Seg{0, s, 1, 0, -1, unit});
+            ++unit;
+            ++s;
+            continue;
+        }
+        std::vector<u64> V;
+        u64 Aun = p->steps[s].in_mask;
+        int e = s;
+        while (e < n && !step_is_resident(p->steps[e], p->resident_bits)) {
+            std::vector<u64> V2 = V;
+            for (u64 m : step_masks(p->steps[e])) rref_insert(V2, m);
+            if ((int)V2.size() > D && e > s) break;
+            V = V2;
+            if (p->steps[e].kind == 0) Aun |= 1ull << p->steps[e].pivot;
+            ++e;
+        }
+        const int d = (int)V.size();
+        u64 piv = 0;
+        for (u64 v : V) piv |= 1ull << top_bit(v);
+        if (piv & ~Aun) return fail(DCS_ERR_INVALID, "group pivots outside live slots at step %d", s);
+        const u64 R = Aun & ~piv;
+        const int rbits = popc(R);
+        int wb = std::min(TL - d, rbits);
+        if (wb < 0) wb = 0;
+        u64 low = 0;
+        {
+            u64 r = R;
+            for (int j = 0; j < wb; ++j) {
+                low |= r & (~r + 1);
+                r &= r - 1;
+            }
+        }
+        GroupDev g;
+        memset(&g, 0, sizeof(g));
+        g.d = d;
+        g.wbits = wb;
+        g.lane_identity = (low == ((wb == 64) ? ~0ull : ((1ull << wb) - 1)));
+        g.nsteps = e - s;
+        g.first_gstep = (int)gsteps.size();
+        g.addrv_offset = (int)addrv.size();
+        g.rep_low_mask = low;
+        g.ntiles = 1ull << (rbits - wb);
+        g.high = make_runs(R & ~low);
+        auto coords = [&](u64 m, uint32_t &out) -> bool {
+            uint32_t c = 0;
+            u64 chk = 0;
+            for (int i = 0; i < d; ++i)
+                if ((m >> top_bit(V[i])) & 1) {
+                    c |= 1u << i;
+                    chk ^= V[i];
+                }
+            out = c;
+            return chk == m;
+        };
+        auto tindex = [&](int slot) -> int {
+            for (int i = 0; i < d; ++i)
+                if (top_bit(V[i]) == slot) return i;
+            return -1;
+        };
+        const u64 dead_in = Aun & ~p->steps[s].in_mask;
+        const u64 dead_out = Aun & ~p->steps[e - 1].out_mask;
+        for (int i = 0; i < d; ++i) {
+            const int slot = top_bit(V[i]);
+            if ((dead_in >> slot) & 1) g.dead_in_t |= 1u << i;
+            if ((dead_out >> slot) & 1) g.dead_out_t |= 1u << i;
+        }
+        if ((dead_in | dead_out) & ~piv)
+            return fail(DCS_ERR_INVALID, "group %d: a dead slot is not a coset pivot", (int)groups.size());
+        for (uint32_t t = 0; t < (1u << d); ++t) {
+            u64 a = 0;
+            for (int i = 0; i < d; ++i)
+                if ((t >> i) & 1) a ^= V[i];
+            addrv.push_back(a);
+        }
+        for (int j = s; j < e; ++j) {
+            const StepHost &st = p->steps[j];
+            GStepDev gs;
+            memset(&gs, 0, sizeof(gs));
+            gs.beta = st.beta;
+            gs.kind = st.kind;
+            gs.step = j;
+            gs.tp = gs.tpa = 0;
+            bool ok = true;
+            if (st.kind == 0) gs.tp = tindex(st.pivot);
+            if (st.kind == 3 || st.kind == 5 || st.kind == 6 || st.kind == 7) {
+                ok = ok && coords(st.K, gs.tk);
+                gs.tp = tindex(st.pivot);
+            }
+            if (st.kind == 1 || st.kind == 6) {
+                ok = ok && coords(st.Ka, gs.tka);
+                gs.tpa = tindex(st.pivot_a);
+                if (st.kind == 1) gs.tp = gs.tpa;
+            }
+            if (!ok || gs.tp < 0 || gs.tpa < 0)
+                return fail(DCS_ERR_INVALID, "step %d: orbit mask outside its group subspace", j);
+            for (int i = 0; i < d; ++i)
+                if (popc(V[i] & st.beta) & 1) gs.bvec |= 1u << i;
+            {
+                u64 r = low;
+                for (int jb = 0; jb < wb; ++jb) {
+                    const u64 bit = r & (~r + 1);
+                    if (st.beta & bit) gs.beta_lane |= 1u << jb;
+                    r &= r - 1;
+                }
+            }
+            for (int i = 0; i < d; ++i)
+                if (!((st.in_mask >> top_bit(V[i])) & 1)) gs.inactive_t |= 1u << i;
+            gsteps.push_back(gs);
+        }
+        p->sched_fused.push_back(Seg{2, s, e - s, d, (int)groups.size(), unit});
+        groups.push_back(g);
+        ++unit;
+        s = e;
+    }
+    p->units_fused = unit;
+    return DCS_OK;
+}
+
+template <class S>
+int launch_step(dcs_plan *p, int nl, int step, int nunits, int unit) {
+    const StepArgs &a = p->step_args[step];
+    dim3 grid(grid_for(a.norbits, p->ctx->cus, 32), nl);
+    S *E = (S *)p->frontier;
+#define L(K)                                                                                       \
+    case K:                                                                                        \
+        PROF(0, (k_step<S, K><<<grid, kThreads, 0, p->ctx->stream>>>(E, p->lane_stride, p->d_lanes, a, p->d_dyn, \
+                                                                     p->nsteps, step, p->d_dmax, nunits, unit))); \
+        break;
+    switch (a.kind) {
+        L(0) L(1) L(2) L(3) L(4) L(5) L(6) L(7)
+        default: return fail(DCS_ERR_INVALID, "bad kind");
+    }
+#undef L
+    return DCS_OK;
+}
+
+template <class S>
+int launch_resident(dcs_plan *p, int nl, const Seg &sg, int nunits) {
+    S *E = (S *)p->frontier;
+    const int rb = p->resident_bits;
+#define R(B)                                                                                         \
+    case B:                                                                                          \
+        PROF(1, (k_resident<S, B><<<nl, kThreads, 0, p->ctx->stream>>>(E, p->lane_stride, p->d_lanes, p->d_steps, \
+                                                                       sg.first, sg.first + sg.count, p->d_dyn, \
+                                                                       p->nsteps, p->d_dmax, nunits, sg.unit))); \
+        break;
+    switch (rb) {
+        R(6) R(7) R(8) R(9) R(10) R(11) R(12)
+        default: return fail(DCS_ERR_INVALID, "resident_bits must be 6..12");
+    }
+#undef R
+    return DCS_OK;
+}
+
+template <class S>
+int launch_coset(dcs_plan *p, int nl, const Seg &sg, int nunits, u64 ntiles) {
+    S *E = (S *)p->frontier;
+    dim3 grid((unsigned)std::max<u64>(1, std::min<u64>(ntiles, (u64)p->ctx->cus * 8)), nl);
+#define C(T)                                                                                         \
+    case T:                                                                                          \
+        PROF(2, (k_coset<S, T><<<grid, kThreads, 0, p->ctx->stream>>>(E, p->lane_stride, p->d_lanes, p->d_groups, \
+                                                                      sg.group, p->d_gsteps, p->d_addrv, p->d_dyn, \
+                                                                      p->nsteps, p->d_dmax, nunits, sg.unit))); \
+        break;
+    switch (p->tile_log2) {
+        C(10) C(11) C(12)
+        default: return fail(DCS_ERR_INVALID, "tile_log2 must be 10..12");
+    }
+#undef C
+    return DCS_OK;
+}
+
+template <class S>
+int run_contract(dcs_plan *p, int mode, int nl, std::vector<u64> &ntiles_of_group) {
+    const std::vector<Seg> &sched = (mode == DCS_MODE_FUSED) ? p->sched_fused : p->sched_steps;
+    const int nunits = (mode == DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
+    {
+        const u64 cnt = (u64)nl * nunits;
+        k_fill_u64<<<grid_for(cnt, p->ctx->cus, 4), kThreads, 0, p->ctx->stream>>>(p->d_dmax, cnt, 0ull);
+        HIPCHK(hipGetLastError());
+        k_init_lanes<S><<<1, 64, 0, p->ctx->stream>>>((S *)p->frontier, p->lane_stride, p->d_lanes, nl);
+        HIPCHK(hipGetLastError());
+    }
+    for (const Seg &sg : sched) {
+        int rc = DCS_OK;
+        if (sg.type == 0) rc = launch_step<S>(p, nl, sg.first, nunits, sg.unit);
+        else if (sg.type == 1) rc = launch_resident<S>(p, nl, sg, nunits);
+        else rc = launch_coset<S>(p, nl, sg, nunits, ntiles_of_group[sg.group]);
+        if (rc) return rc;
+        if (mode == DCS_MODE_STEPS_HOST) {
+            // The reference driver reads the maximum back after every step (inplace_gpu.py:255-258).
+            HIPCHK(hipMemcpyAsync(p->h_dmax, p->d_dmax, sizeof(u64) * (size_t)nl * nunits, hipMemcpyDeviceToHost,
+                                  p->ctx->stream));
+            HIPCHK(hipStreamSynchronize(p->ctx->stream));
+            bool all_zero = true, bad = false;
+            for (int li = 0; li < nl; ++li) {
+                double m2;
+                const u64 b = p->h_dmax[(size_t)li * nunits + sg.unit];
+                memcpy(&m2, &b, 8);
+                if (!std::isfinite(m2)) bad = true;
+                if (m2 != 0.0) all_zero = false;
+            }
+            if (bad || all_zero) break;
+        }
+    }
+    return DCS_OK;
+}
+
+}  // namespace
+
+// ================================================================================================
+extern "C" {
+
+const char *dcs_last_error(void) { return g_error.c_str(); }
+
+int dcs_get_device_count(int *count) {
+    if (!count) return fail(DCS_ERR_INVALID, "null count");
+    HIPCHK(hipGetDeviceCount(count));
+    return DCS_OK;
+}
+
+int dcs_get_device_info(int device, dcs_device_info *out) {
+    if (!out) return fail(DCS_ERR_INVALID, "null out");
+    memset(out, 0, sizeof(*out));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    snprintf(out->name, sizeof(out->name), "%s", prop.name);
+    snprintf(out->arch, sizeof(out->arch), "%s", prop.gcnArchName);
+    out->compute_units = prop.multiProcessorCount;
+    HIPCHK(hipSetDevice(device));
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    out->free_bytes = fr;
+    out->total_bytes = tot;
+    HIPCHK(hipRuntimeGetVersion(&out->runtime_version));
+    HIPCHK(hipDriverGetVersion(&out->driver_version));
+    out->device = device;
+    return DCS_OK;
+}
+
+int dcs_context_create(int device, dcs_context **out) {
+    if (!out) return fail(DCS_ERR_INVALID, "null out");
+    *out = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(DCS_ERR_INVALID, "device %d out of range (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    dcs_context *c = new dcs_context();
+    c->device = device;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    c->cus = prop.multiProcessorCount;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = c;
+    return DCS_OK;
+}
+
+void dcs_context_destroy(dcs_context *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int dcs_synchronize(dcs_context *ctx) {
+    if (!ctx) return fail(DCS_ERR_INVALID, "null context");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DCS_OK;
+}
+
+void dcs_plan_destroy(dcs_plan *p) {
+    if (!p) return;
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    for (auto &pe : p->pending) {
+        (void)hipEventDestroy(pe.a);
+        (void)hipEventDestroy(pe.b);
+    }
+    void *dev[] = {p->frontier, p->d_dyn, p->d_lanes, p->d_dmax, p->d_steps, p->d_groups, p->d_gsteps,
+                   p->d_addrv, p->d_smap, p->d_phase, p->d_blocks, p->d_cum, p->d_scratch, p->d_draw};
+    for (void *q : dev)
+        if (q) (void)hipFree(q);
+    void *host[] = {p->h_dyn, p->h_lanes, p->h_dmax, p->h_draw};
+    for (void *q : host)
+        if (q) (void)hipHostFree(q);
+    delete p;
+}
+
+int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
+    if (!ctx || !d || !out) return fail(DCS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (d->storage != DCS_STORAGE_C64 && d->storage != DCS_STORAGE_C128)
+        return fail(DCS_ERR_INVALID, "storage must be 0 (complex64) or 1 (complex128)");
+    if (d->maxslots < 0 || d->maxslots > 40) return fail(DCS_ERR_INVALID, "maxslots %d out of range", d->maxslots);
+    if (d->lanes < 1 || d->nsteps < 1 || d->blocks < 1) return fail(DCS_ERR_INVALID, "lanes, nsteps, blocks must be >= 1");
+    if (d->k < 0 || d->k > 30 || popc(d->terminal_mask) > d->k)
+        return fail(DCS_ERR_INVALID, "terminal mask holds more than k slots (k=%d)", d->k);
+    if (d->tile_log2 < 10 || d->tile_log2 > 12) return fail(DCS_ERR_INVALID, "tile_log2 must be 10..12");
+    if (d->resident_bits != 0 && (d->resident_bits < 6 || d->resident_bits > 12))
+        return fail(DCS_ERR_INVALID, "resident_bits must be 0 or 6..12");
+    if (d->fuse_max_dim < 0 || d->fuse_max_dim > kMaxDim) return fail(DCS_ERR_INVALID, "fuse_max_dim must be 0..8");
+    const u64 lim = (d->maxslots == 0) ? 1ull : ((1ull << d->maxslots) - 1);
+    if (d->terminal_mask & ~lim) return fail(DCS_ERR_INVALID, "terminal slots beyond maxslots");
+    HIPCHK(hipSetDevice(ctx->device));
+
+    dcs_plan *p = new dcs_plan();
+    p->ctx = ctx;
+    p->storage = d->storage;
+    p->maxslots = d->maxslots;
+    p->lanes = d->lanes;
+    p->nsteps = d->nsteps;
+    p->k = d->k;
+    p->blocks = d->blocks;
+    p->fuse_max_dim = d->fuse_max_dim;
+    p->resident_bits = d->resident_bits;
+    p->tile_log2 = d->tile_log2;
+    p->esz = d->storage == DCS_STORAGE_C64 ? sizeof(float2) : sizeof(double2);
+    p->lane_stride = 1ull << d->maxslots;
+    p->terminal_mask = d->terminal_mask;
+    p->unitaries.assign(d->open_unitaries, d->open_unitaries + (size_t)d->k * 8);
+
+    auto bail = [&](int rc) {
+        dcs_plan_destroy(p);
+        return rc;
+    };
+    std::vector<StepDev> sdev(d->nsteps);
+    for (int s = 0; s < d->nsteps; ++s) {
+        StepHost h;
+        h.kind = d->kind[s];
+        h.pivot = d->pivot[s];
+        h.pivot_a = d->pivot_a[s];
+        h.K = d->K[s];
+        h.Ka = d->Ka[s];
+        h.beta = d->beta[s];
+        h.in_mask = d->in_mask[s];
+        h.out_mask = d->out_mask[s];
+        if (h.kind < 0 || h.kind > 7) return bail(fail(DCS_ERR_INVALID, "step %d: kind %d", s, h.kind));
+        if ((h.in_mask | h.out_mask | h.K | h.Ka | h.beta) & ~lim)
+            return bail(fail(DCS_ERR_INVALID, "step %d: mask beyond maxslots", s));
+        const bool needK = h.kind == 3 || h.kind == 5 || h.kind == 6 || h.kind == 7;
+        const bool needKa = h.kind == 1 || h.kind == 6;
+        if (needK && (!h.K || top_bit(h.K) != h.pivot || (h.K & ~h.in_mask)))
+            return bail(fail(DCS_ERR_INVALID, "step %d: K / pivot inconsistent", s));
+        if (needKa && (!h.Ka || top_bit(h.Ka) != h.pivot_a || (h.Ka & ~h.in_mask)))
+            return bail(fail(DCS_ERR_INVALID, "step %d: Ka / pivot_a inconsistent", s));
+        if (h.kind == 6 && ((h.Ka >> h.pivot) & 1)) return bail(fail(DCS_ERR_INVALID, "step %d: Ka holds the K pivot", s));
+        if (h.kind == 0 && (h.pivot < 0 || h.pivot >= d->maxslots || ((h.in_mask >> h.pivot) & 1)))
+            return bail(fail(DCS_ERR_INVALID, "step %d: grow slot invalid", s));
+        if (s > 0 && h.in_mask != p->steps.back().out_mask)
+            return bail(fail(DCS_ERR_INVALID, "step %d: live slots do not chain", s));
+        if (s == 0 && h.in_mask != 0) return bail(fail(DCS_ERR_INVALID, "first step must start from the empty basis"));
+        h.slots = orbit_slots(h);
+        p->steps.push_back(h);
+        StepArgs a;
+        memset(&a, 0, sizeof(a));
+        a.norbits = 1ull << popc(h.slots);
+        a.K = h.K;
+        a.Ka = h.Ka;
+        a.beta = h.beta;
+        a.kind = h.kind;
+        a.pivot = h.pivot;
+        a.runs = make_runs(h.slots);
+        p->step_args.push_back(a);
+        StepDev &sd = sdev[s];
+        sd.K = h.K;
+        sd.Ka = h.Ka;
+        sd.beta = h.beta;
+        sd.slots = h.slots;
+        sd.in_mask = h.in_mask;
+        sd.out_mask = h.out_mask;
+        sd.kind = h.kind;
+        sd.pivot = h.pivot;
+        sd.pivot_a = h.pivot_a;
+        sd.nslots = popc(h.slots);
+    }
+    if (p->steps.back().out_mask != d->terminal_mask)
+        return bail(fail(DCS_ERR_INVALID, "terminal slots differ from the live slots after the last step"));
+
+    std::vector<GroupDev> groups;
+    std::vector<GStepDev> gsteps;
+    std::vector<u64> addrv;
+    {
+        const int rc = build_schedules(p, groups, gsteps, addrv);
+        if (rc) return bail(rc);
+    }
+
+    // Device memory admission: the frontier must fit what is free now.
+    const u64 frontier_bytes = (u64)d->lanes * p->lane_stride * p->esz;
+    const u64 block_len = 1ull << d->k;
+    const u64 aux_bytes = block_len * (8 + 8 + 16 * (u64)d->blocks + 8 + 16) + (64ull << 20);
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return bail(fail(DCS_ERR_RUNTIME, "hipMemGetInfo failed"));
+    if (frontier_bytes + aux_bytes > fr)
+        return bail(fail(DCS_ERR_NOMEM, "frontier needs %llu bytes (+%llu auxiliary), %llu free", (unsigned long long)frontier_bytes,
+                         (unsigned long long)aux_bytes, (unsigned long long)fr));
+#define DALLOC(ptr, bytes)                                                                       \
+    do {                                                                                         \
+        if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess)                                   \
+            return bail(fail(DCS_ERR_NOMEM, "hipMalloc of %llu bytes failed", (unsigned long long)(bytes))); \
+    } while (0)
+#define HALLOC(ptr, bytes)                                                                       \
+    do {                                                                                         \
+        if (hipHostMalloc((void **)&(ptr), (bytes), hipHostMallocDefault) != hipSuccess)         \
+            return bail(fail(DCS_ERR_NOMEM, "hipHostMalloc of %llu bytes failed", (unsigned long long)(bytes))); \
+    } while (0)
+    const int max_units = std::max(p->units_steps, p->units_fused);
+    DALLOC(p->frontier, frontier_bytes);
+    DALLOC(p->d_dyn, sizeof(Dyn) * (size_t)d->lanes * d->nsteps);
+    DALLOC(p->d_lanes, sizeof(int) * (size_t)d->lanes);
+    DALLOC(p->d_dmax, sizeof(u64) * (size_t)d->lanes * max_units);
+    DALLOC(p->d_steps, sizeof(StepDev) * sdev.size());
+    DALLOC(p->d_groups, sizeof(GroupDev) * std::max<size_t>(1, groups.size()));
+    DALLOC(p->d_gsteps, sizeof(GStepDev) * std::max<size_t>(1, gsteps.size()));
+    DALLOC(p->d_addrv, sizeof(u64) * std::max<size_t>(1, addrv.size()));
+    DALLOC(p->d_smap, sizeof(u64) * block_len);
+    DALLOC(p->d_phase, sizeof(double) * block_len);
+    DALLOC(p->d_blocks, sizeof(double2) * block_len * d->blocks);
+    DALLOC(p->d_cum, sizeof(double) * block_len);
+    DALLOC(p->d_scratch, sizeof(double2) * block_len);
+    DALLOC(p->d_draw, sizeof(DrawOut));
+    HALLOC(p->h_dyn, sizeof(Dyn) * (size_t)d->lanes * d->nsteps);
+    HALLOC(p->h_lanes, sizeof(int) * (size_t)d->lanes);
+    HALLOC(p->h_dmax, sizeof(u64) * (size_t)d->lanes * max_units);
+    HALLOC(p->h_draw, sizeof(DrawOut));
+#undef DALLOC
+#undef HALLOC
+    hipStream_t st = ctx->stream;
+#define H2D(dst, src, bytes)                                                                   \
+    do {                                                                                       \
+        if (hipMemcpyAsync((dst), (src), (bytes), hipMemcpyHostToDevice, st) != hipSuccess)    \
+            return bail(fail(DCS_ERR_RUNTIME, "upload failed"));                               \
+    } while (0)
+    H2D(p->d_steps, sdev.data(), sizeof(StepDev) * sdev.size());
+    if (!groups.empty()) H2D(p->d_groups, groups.data(), sizeof(GroupDev) * groups.size());
+    if (!gsteps.empty()) H2D(p->d_gsteps, gsteps.data(), sizeof(GStepDev) * gsteps.size());
+    if (!addrv.empty()) H2D(p->d_addrv, addrv.data(), sizeof(u64) * addrv.size());
+    H2D(p->d_smap, d->syndrome_map, sizeof(u64) * ((u64)1 << popc(d->terminal_mask)));
+    H2D(p->d_phase, d->phase, sizeof(double) * block_len);
+#undef H2D
+    if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(DCS_ERR_RUNTIME, "plan upload sync failed"));
+    p->group_ntiles.resize(groups.size());
+    for (size_t i = 0; i < groups.size(); ++i) p->group_ntiles[i] = groups[i].ntiles;
+    *out = p;
+    return DCS_OK;
+}
+
+
+int dcs_plan_units(dcs_plan *p, int mode, int *nunits) {
+    if (!p || !nunits) return fail(DCS_ERR_INVALID, "null argument");
+    if (mode < 0 || mode > 2) return fail(DCS_ERR_INVALID, "mode %d", mode);
+    *nunits = (mode == DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
+    return DCS_OK;
+}
+
+int dcs_plan_schedule(dcs_plan *p, int mode, int cap, int32_t *seg_kind, int32_t *first, int32_t *count,
+                      int32_t *dim, int *nseg) {
+    if (!p || !nseg) return fail(DCS_ERR_INVALID, "null argument");
+    if (mode < 0 || mode > 2) return fail(DCS_ERR_INVALID, "mode %d", mode);
+    const std::vector<Seg> &s = (mode == DCS_MODE_FUSED) ? p->sched_fused : p->sched_steps;
+    *nseg = (int)s.size();
+    for (int i = 0; i < (int)s.size() && i < cap; ++i) {
+        if (seg_kind) seg_kind[i] = s[i].type;
+        if (first) first[i] = s[i].first;
+        if (count) count[i] = s[i].count;
+        if (dim) dim[i] = s[i].dim;
+    }
+    return DCS_OK;
+}
+
+int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const double *coeffs, const int32_t *ov,
+                 const int32_t *y, double *unit_m2) {
+    if (!p || !lanes || !coeffs || !ov || !y || !unit_m2) return fail(DCS_ERR_INVALID, "null argument");
+    if (mode < 0 || mode > 2) return fail(DCS_ERR_INVALID, "mode %d", mode);
+    if (nl < 1 || nl > p->lanes) return fail(DCS_ERR_INVALID, "nlanes %d outside 1..%d", nl, p->lanes);
+    for (int i = 0; i < nl; ++i) {
+        if (lanes[i] < 0 || lanes[i] >= p->lanes) return fail(DCS_ERR_INVALID, "lane %d out of range", lanes[i]);
+        for (int j = 0; j < i; ++j)
+            if (lanes[j] == lanes[i]) return fail(DCS_ERR_INVALID, "lane %d repeated", lanes[i]);
+    }
+    HIPCHK(hipSetDevice(p->ctx->device));
+    const int n = p->nsteps;
+    for (int li = 0; li < nl; ++li)
+        for (int s = 0; s < n; ++s) {
+            Dyn &d = p->h_dyn[(size_t)li * n + s];
+            const double *c = coeffs + ((size_t)li * n + s) * 4;
+            d.f0r = c[0];
+            d.f0i = c[1];
+            d.f1r = c[2];
+            d.f1i = c[3];
+            d.ov = ov[(size_t)li * n + s] & 1;
+            d.y = y[(size_t)li * n + s] & 1;
+            d.pad[0] = d.pad[1] = 0;
+        }
+    for (int i = 0; i < nl; ++i) p->h_lanes[i] = lanes[i];
+    hipStream_t st = p->ctx->stream;
+    HIPCHK(hipMemcpyAsync(p->d_dyn, p->h_dyn, sizeof(Dyn) * (size_t)nl * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->d_lanes, p->h_lanes, sizeof(int) * nl, hipMemcpyHostToDevice, st));
+    const int rc = (p->storage == DCS_STORAGE_C64) ? run_contract<float2>(p, mode, nl, p->group_ntiles)
+                                                   : run_contract<double2>(p, mode, nl, p->group_ntiles);
+    if (rc) {
+        (void)hipStreamSynchronize(st);
+        return rc;
+    }
+    const int nunits = (mode == DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
+    HIPCHK(hipMemcpyAsync(p->h_dmax, p->d_dmax, sizeof(u64) * (size_t)nl * nunits, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    memcpy(unit_m2, p->h_dmax, sizeof(double) * (size_t)nl * nunits);
+    return ev_resolve(p);
+}
+
+int dcs_block_zero(dcs_plan *p, int block) {
+    if (!p || block < 0 || block >= p->blocks) return fail(DCS_ERR_INVALID, "block out of range");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    const u64 n = 2ull << p->k;  // doubles
+    PROF(3, (k_fill_u64<<<grid_for(n, p->ctx->cus, 8), kThreads, 0, p->ctx->stream>>>(
+                 (u64 *)(p->d_blocks + ((u64)block << p->k)), n, 0ull)));
+    return DCS_OK;
+}
+
+int dcs_block_accumulate(dcs_plan *p, int block, int lane, uint64_t xmask, double cr, double ci) {
+    if (!p || block < 0 || block >= p->blocks) return fail(DCS_ERR_INVALID, "block out of range");
+    if (lane < 0 || lane >= p->lanes) return fail(DCS_ERR_INVALID, "lane out of range");
+    if (xmask >= (1ull << p->k)) return fail(DCS_ERR_INVALID, "xmask outside the block");
+    const u64 n = 1ull << popc(p->terminal_mask);
+    HIPCHK(hipSetDevice(p->ctx->device));
+    const Runs term = make_runs(p->terminal_mask);
+    double2 *tot = p->d_blocks + ((u64)block << p->k);
+    const unsigned g = grid_for(n, p->ctx->cus, 8);
+    if (p->storage == DCS_STORAGE_C64) {
+        const float2 *E = (const float2 *)p->frontier + (u64)lane * p->lane_stride;
+        PROF(3, (k_scatter<float2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, p->d_smap, xmask, cd{cr, ci}, tot)));
+    } else {
+        const double2 *E = (const double2 *)p->frontier + (u64)lane * p->lane_stride;
+        PROF(3, (k_scatter<double2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, p->d_smap, xmask, cd{cr, ci}, tot)));
+    }
+    return DCS_OK;
+}
+
+int dcs_block_transform(dcs_plan *p, int block) {
+    if (!p || block < 0 || block >= p->blocks) return fail(DCS_ERR_INVALID, "block out of range");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    const u64 n = 1ull << p->k;
+    double2 *v = p->d_blocks + ((u64)block << p->k);
+    const unsigned gh = grid_for(std::max<u64>(1, n / 2), p->ctx->cus, 8);
+    const unsigned gn = grid_for(n, p->ctx->cus, 8);
+    for (int q = 0; q < p->k; ++q) PROF(3, (k_wht_pass<<<gh, kThreads, 0, p->ctx->stream>>>(v, n / 2, q)));
+    PROF(3, (k_phase<<<gn, kThreads, 0, p->ctx->stream>>>(v, p->d_phase, n)));
+    for (int q = 0; q < p->k; ++q) {
+        const double *u = &p->unitaries[(size_t)q * 8];
+        PROF(3, (k_u_pass<<<gh, kThreads, 0, p->ctx->stream>>>(v, n / 2, q, cd{u[0], u[1]}, cd{u[2], u[3]},
+                                                               cd{u[4], u[5]}, cd{u[6], u[7]})));
+    }
+    return DCS_OK;
+}
+
+int dcs_block_read(dcs_plan *p, int block, double *host_out) {
+    if (!p || !host_out || block < 0 || block >= p->blocks) return fail(DCS_ERR_INVALID, "bad argument");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipMemcpyAsync(host_out, p->d_blocks + ((u64)block << p->k), sizeof(double2) << p->k, hipMemcpyDeviceToHost,
+                          p->ctx->stream));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    return ev_resolve(p);
+}
+
+int dcs_block_device_pointer(dcs_plan *p, int block, void **ptr) {
+    if (!p || !ptr || block < 0 || block >= p->blocks) return fail(DCS_ERR_INVALID, "bad argument");
+    *ptr = (void *)(p->d_blocks + ((u64)block << p->k));
+    return DCS_OK;
+}
+
+int dcs_draw(dcs_plan *p, int block, double u, dcs_draw_result *out) {
+    if (!p || !out || block < 0 || block >= p->blocks) return fail(DCS_ERR_INVALID, "bad argument");
+    if (!(u >= 0.0 && u < 1.0)) return fail(DCS_ERR_INVALID, "uniform must lie in [0, 1)");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    const u64 n = 1ull << p->k;
+    const double2 *v = p->d_blocks + ((u64)block << p->k);
+    PROF(4, (k_cumsum<<<1, 64, 0, p->ctx->stream>>>(v, n, p->d_cum)));
+    PROF(4, (k_search<<<1, 64, 0, p->ctx->stream>>>(v, p->d_cum, n, u, p->d_draw)));
+    HIPCHK(hipMemcpyAsync(p->h_draw, p->d_draw, sizeof(DrawOut), hipMemcpyDeviceToHost, p->ctx->stream));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    int rc = ev_resolve(p);
+    if (rc) return rc;
+    const DrawOut &r = *p->h_draw;
+    out->index = r.index;
+    out->weight = r.weight;
+    out->total = r.total;
+    out->cdf_lower = r.lower;
+    out->cdf_upper = r.upper;
+    out->margin = r.margin;
+    if (r.status == DCS_ERR_NONFINITE) return fail(DCS_ERR_NONFINITE, "nonfinite block mass");
+    if (r.status != DCS_OK) return fail(DCS_ERR_ZERO, "numerically zero block or invalid CDF outcome");
+    return DCS_OK;
+}
+
+int dcs_frontier_read(dcs_plan *p, int lane, void *host_out) {
+    if (!p || !host_out || lane < 0 || lane >= p->lanes) return fail(DCS_ERR_INVALID, "bad argument");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    const u64 n = 1ull << popc(p->terminal_mask);
+    const Runs term = make_runs(p->terminal_mask);
+    const unsigned g = grid_for(n, p->ctx->cus, 8);
+    if (p->storage == DCS_STORAGE_C64) {
+        const float2 *E = (const float2 *)p->frontier + (u64)lane * p->lane_stride;
+        k_gather_terminal<float2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, (float2 *)p->d_scratch);
+    } else {
+        const double2 *E = (const double2 *)p->frontier + (u64)lane * p->lane_stride;
+        k_gather_terminal<double2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, (double2 *)p->d_scratch);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host_out, p->d_scratch, p->esz * n, hipMemcpyDeviceToHost, p->ctx->stream));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    return DCS_OK;
+}
+
+int dcs_plan_profile(dcs_plan *p, int enable, double *ms, int64_t *launches) {
+    if (!p) return fail(DCS_ERR_INVALID, "null plan");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    int rc = ev_resolve(p);
+    if (rc) return rc;
+    for (int i = 0; i < 5; ++i) {
+        if (ms) ms[i] = p->prof_ms[i];
+        if (launches) launches[i] = p->prof_n[i];
+        p->prof_ms[i] = 0;
+        p->prof_n[i] = 0;
+    }
+    p->profiling = enable != 0;
+    return DCS_OK;
+}
+
+int dcs_stream_copy(dcs_context *ctx, uint64_t bytes, int reps, double *ms) {
+    if (!ctx || !ms || reps < 1 || bytes < 16 || (bytes & 15)) return fail(DCS_ERR_INVALID, "bad argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    double2 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void **)&a, bytes) != hipSuccess) return fail(DCS_ERR_NOMEM, "stream copy source");
+    if (hipMalloc((void **)&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(DCS_ERR_NOMEM, "stream copy destination");
+    }
+    const u64 n = bytes / 16;
+    const unsigned g = grid_for(n, ctx->cus, 64);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemsetAsync(a, 0, bytes, ctx->stream));
+    k_stream_copy<<<g, kThreads, 0, ctx->stream>>>(a, b, n);
+    HIPCHK(hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < reps; ++r) k_stream_copy<<<g, kThreads, 0, ctx->stream>>>(a, b, n);
+    HIPCHK(hipEventRecord(e1, ctx->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return DCS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+"""Write ibm_qat245_64q.npz: the frozen 64-qubit QAT #245 injected graph and its slice plans.
+
+Source files (read-only, in the reference checkout): inputs/injected_graph.npz (graph u8[378,378],
+unitaries c128[378,2,2], order, ring, natural_order) and inputs/plans.json (open vertices and the
+parity-operation lists of every plan).  The adjacency is stored bit-packed; nothing else changes.
+
+Usage: python extract_ibm.py /root/reference
+"""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+
+def main(ref):
+    ref = Path(ref)
+    src = np.load(ref / 'inputs/injected_graph.npz', allow_pickle=False)
+    plans = json.loads((ref / 'inputs/plans.json').read_text())
+    graph = src['graph'].astype(np.uint8)
+    assert (graph == graph.T).all() and not graph.diagonal().any()
+    out = {
+        'graph_packed': np.packbits(graph, axis=1),
+        'N': np.int64(graph.shape[0]),
+        'unitaries': src['unitaries'].astype(np.complex128),
+        'order': src['order'].astype(np.int64),
+        'ring': src['ring'].astype(np.int64),
+        'natural_order': src['natural_order'].astype(np.int64),
+        'open_vertices': np.asarray(plans['open_vertices'], np.int64),
+        'plans_json': np.frombuffer(json.dumps(plans['plans']).encode(), np.uint8),
+    }
+    dst = Path(__file__).resolve().parent / 'ibm_qat245_64q.npz'
+    np.savez_compressed(dst, **out)
+    print(dst, dst.stat().st_size)
+
+
+if __name__ == '__main__':
+    main(sys.argv[1] if len(sys.argv) > 1 else '/root/reference')

@@ -1,0 +1,151 @@
+/*
+ * dcs_amd.h -- C ABI of the MI355X frontier-contraction library (libdcs_amd.so).
+ *
+ * The library executes the preserved-coordinate ("in-place") contraction of a parity-sliced graph
+ * state on one gfx950 GPU, accumulates the coherent conditional block, applies the Walsh-Hadamard /
+ * phase / local-Clifford tail and draws from the |a|^2 CDF.  All device memory is owned by the
+ * library; the host passes plain arrays.  Every entry point returns a dcs_status; the message of the
+ * last failure on the calling thread is available from dcs_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   dcs_context_create        cp.cuda.Device(device).use()            kernel_opt/inplace_gpu.py:221-222
+ *   dcs_get_device_info       getDeviceProperties / runtime versions  production/worker.py:60-76
+ *   dcs_plan_create           InplaceGPUMixin.alloc (plan upload,     kernel_opt/inplace_gpu.py:220-240
+ *                             frontier, syndrome map, phase)          sampling/conditional_block_inplace_gpu.py:195-211
+ *   dcs_contract              InplaceGPUMixin.prefix_inplace           kernel_opt/inplace_gpu.py:242-259
+ *                             (one launch of inplace_step per step)   kernel_opt/inplace_gpu.py:155-198
+ *   dcs_block_zero,
+ *   dcs_block_accumulate      total = zeros; total[indices] += env*coef  sampling/conditional_block_gpu.py:137,155-158
+ *   dcs_block_transform       Walsh-Hadamard, phase, per-qubit U      sampling/conditional_block_gpu.py:159-171
+ *   dcs_block_read            cp.asnumpy(vector)                      production/worker.py:109
+ *   dcs_draw                  draw_checked primary CDF (cumsum,       sampling/sampling_protocol.py:56-65,81-82
+ *                             searchsorted side='right')
+ *   dcs_frontier_read         d_inplace[d_inplace_terminal_indices]   kernel_opt/inplace_gpu.py:237-239,259
+ */
+#ifndef DCS_AMD_H
+#define DCS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DCS_OK = 0,
+    DCS_ERR_INVALID = 1,    /* bad argument or plan (Python: ValueError) */
+    DCS_ERR_RUNTIME = 2,    /* HIP runtime failure (RuntimeError) */
+    DCS_ERR_NOMEM = 3,      /* frontier does not fit the free device memory (MemoryError) */
+    DCS_ERR_NONFINITE = 4,  /* nonfinite frontier or block (ArithmeticError) */
+    DCS_ERR_ZERO = 5        /* zero / invalid CDF outcome (ArithmeticError) */
+} dcs_status;
+
+typedef struct dcs_context dcs_context;
+typedef struct dcs_plan dcs_plan;
+
+typedef struct {
+    char name[256];
+    char arch[64];
+    uint64_t total_bytes;
+    uint64_t free_bytes;
+    int32_t compute_units;
+    int32_t runtime_version;
+    int32_t driver_version;
+    int32_t device;
+} dcs_device_info;
+
+/* Frontier storage of the contraction.  Arithmetic is FP64 in both cases. */
+enum { DCS_STORAGE_C64 = 0, DCS_STORAGE_C128 = 1 };
+
+/* Execution schedules of dcs_contract. */
+enum {
+    DCS_MODE_STEPS = 0,       /* one launch per step, scale chain kept on the device, no host sync */
+    DCS_MODE_STEPS_HOST = 1,  /* one launch per step, the host reads the maximum after every step */
+    DCS_MODE_FUSED = 2        /* LDS-resident runs + fused coset groups (the production schedule) */
+};
+
+typedef struct {
+    int32_t storage;            /* DCS_STORAGE_* */
+    int32_t maxslots;           /* frontier address bits (physical slots), <= 40 */
+    int32_t lanes;              /* independent frontiers held at once */
+    int32_t nsteps;
+    /* per-step in-place plan (orbit code 0..7, masks over physical slots) */
+    const int32_t *kind;
+    const uint64_t *K;
+    const uint64_t *Ka;
+    const int32_t *pivot;       /* grow slot, or highest bit of K (-1 when K == 0) */
+    const int32_t *pivot_a;     /* highest bit of Ka (-1 when Ka == 0) */
+    const uint64_t *beta;       /* sign mask over slots */
+    const uint64_t *in_mask;    /* slots live before the step */
+    const uint64_t *out_mask;   /* slots live after the step */
+    /* conditional block */
+    int32_t k;                  /* log2 of the block length (open vertices) */
+    uint64_t terminal_mask;     /* physical slots of the terminal basis; popcount = terminal rank r <= k */
+    const uint64_t *syndrome_map;   /* [2^r] block index of terminal coordinate i */
+    const double *phase;            /* [2^k] +-1 from edges among open vertices */
+    const double *open_unitaries;   /* [k][8] u00r,u00i,u01r,u01i,u10r,u10i,u11r,u11i */
+    int32_t blocks;             /* complex128 block accumulators */
+    /* schedule knobs (DCS_MODE_FUSED) */
+    int32_t fuse_max_dim;       /* largest coset dimension of a fused group, 1..8 */
+    int32_t resident_bits;      /* steps whose live slots all sit below this bit run LDS-resident, <= 12 */
+    int32_t tile_log2;          /* entries per coset tile, 10..13 */
+} dcs_plan_desc;
+
+typedef struct {
+    int64_t index;          /* searchsorted(cdf, u, side='right') */
+    double weight;          /* |a_index|^2 */
+    double total;           /* sequential cumsum total (float64) */
+    double cdf_lower;       /* normalised cdf[index-1] (0 for index 0) */
+    double cdf_upper;       /* normalised cdf[index] (cdf[-1] forced to 1) */
+    double margin;          /* min(u - lower, upper - u) */
+} dcs_draw_result;
+
+const char *dcs_last_error(void);
+int dcs_get_device_count(int *count);
+int dcs_get_device_info(int device, dcs_device_info *out);
+
+int dcs_context_create(int device, dcs_context **out);
+void dcs_context_destroy(dcs_context *ctx);
+int dcs_synchronize(dcs_context *ctx);
+
+int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *desc, dcs_plan **out);
+void dcs_plan_destroy(dcs_plan *plan);
+/* Number of scale units (maxima) a contraction in `mode` produces per lane. */
+int dcs_plan_units(dcs_plan *plan, int mode, int *nunits);
+/* Schedule of `mode`: per segment (kind 0 step, 1 resident run, 2 coset group), first step, step
+ * count and coset dimension.  Arrays of capacity `cap`; *count receives the segment count. */
+int dcs_plan_schedule(dcs_plan *plan, int mode, int cap, int32_t *seg_kind, int32_t *first,
+                      int32_t *count, int32_t *dim, int *nseg);
+
+/* Contract `nlanes` frontiers (lane ids in `lanes`) through every step.
+ *   coeffs [nlanes][nsteps][4]  f0r, f0i, f1r, f1i of the step's vertex
+ *   ov, y  [nlanes][nsteps]     offset parity bit and forced value of each step
+ *   unit_m2 out [nlanes][nunits] max |out|^2 of every scale unit, in order
+ * Returns after the device finished. */
+int dcs_contract(dcs_plan *plan, int mode, int nlanes, const int32_t *lanes, const double *coeffs,
+                 const int32_t *ov, const int32_t *y, double *unit_m2);
+
+int dcs_block_zero(dcs_plan *plan, int block);
+/* total[smap[i] ^ xmask] += env_lane(i) * coef, env at the terminal slots of the lane. */
+int dcs_block_accumulate(dcs_plan *plan, int block, int lane, uint64_t xmask, double coef_re,
+                         double coef_im);
+int dcs_block_transform(dcs_plan *plan, int block);
+int dcs_block_read(dcs_plan *plan, int block, double *host_out);
+int dcs_block_device_pointer(dcs_plan *plan, int block, void **ptr);
+int dcs_draw(dcs_plan *plan, int block, double u, dcs_draw_result *out);
+/* Terminal environment of a lane (2^r entries in storage precision, interleaved re/im). */
+int dcs_frontier_read(dcs_plan *plan, int lane, void *host_out);
+
+/* Elapsed device time (ms) and launch count per kernel family since the last reset:
+ * index 0 step kernels, 1 resident runs, 2 coset groups, 3 block kernels, 4 draw. */
+int dcs_plan_profile(dcs_plan *plan, int enable, double *ms, int64_t *launches);
+
+/* Device bandwidth probe: copy `bytes` (multiple of 16) `reps` times with a streaming kernel and
+ * return the average kernel time in ms. */
+int dcs_stream_copy(dcs_context *ctx, uint64_t bytes, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

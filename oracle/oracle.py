@@ -1,0 +1,232 @@
+"""Oracle of the sampling hot path.  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py import this module, and only
+as the checker (or, for cpu_baseline, as the timed CPU port).  The product path never calls it.
+
+What each function restates and what pins it:
+
+  contract_inplace   C restatement (liboracle.so) of execute_inplace (kernel_opt/inplace_plan.py:90-135)
+                     with the GPU driver's scale chain (kernel_opt/inplace_gpu.py:242-258).
+                     Pinned by: the dense state vector of the Clifford+T circuit (statevector() below,
+                     closed-form gate matrices) through block(); by gather_amplitude(), an independent
+                     restatement of the other reference executor (dcs_rankwidth/scan_plan.py:161-195)
+                     on the same graph; by the frozen IBM plan checksums (rmax, work per slice).
+  block              slice loop, scatter, Walsh-Hadamard / phase / U tail
+                     (sampling/conditional_block_gpu.py:128-177).  Pinned by statevector() (probabilities
+                     and amplitudes up to one global phase, every prefix).
+  draw               weights, sequential cumsum, searchsorted(side='right') (sampling_protocol.py:56-65).
+                     Pinned by the reference self-test vectors (sampling_protocol.py:103-117) and brute
+                     force interval search.
+  statevector        dense simulation of the ring circuit (H, S, CZ, T = diag(1, e^{i pi/4})); the
+                     mathematical ground truth, pinned by gate identities in the tests.
+  gather_amplitude   numpy restatement of sweep_reference / amplitude_reference (scan_plan.py:161-207).
+  graph_amplitude    brute-force graph-state sum (scan_plan.py:6-8) for N <= 20.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / 'liboracle.so'
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f'{LIB_PATH} missing: run make -C oracle')
+        L = C.CDLL(str(LIB_PATH))
+        P = C.POINTER
+        u64p, i32p, dp = P(C.c_uint64), P(C.c_int32), P(C.c_double)
+        L.oracle_contract.restype = C.c_int
+        L.oracle_contract.argtypes = [C.c_int, C.c_int, i32p, u64p, u64p, i32p, i32p, u64p, u64p, dp, i32p, i32p,
+                                      C.c_void_p, dp]
+        L.oracle_accumulate.restype = None
+        L.oracle_accumulate.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_int, u64p, C.c_uint64, C.c_double,
+                                        C.c_double, dp]
+        L.oracle_transform.restype = None
+        L.oracle_transform.argtypes = [C.c_int, dp, dp, dp]
+        L.oracle_draw.restype = C.c_int
+        L.oracle_draw.argtypes = [dp, C.c_uint64, C.c_double, dp, dp]
+        L.oracle_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def frexp_scale_chain(m2):
+    """ls and liveness of one slice from its per-step maxima (inplace_gpu.py:250-258)."""
+    ls, maximum = 0., 1.
+    for m in m2:
+        ls += math.frexp(maximum)[1] * math.log(2.)
+        if not np.isfinite(m):
+            raise ArithmeticError('Nonfinite in-place environment')
+        if m == 0:
+            return ls, False
+        maximum = math.sqrt(float(m))
+    return ls, True
+
+
+def contract_inplace(bp, f, yfix, storage=1, buffer=None):
+    """Run the in-place plan of BlockPlan `bp` for one slice.  Returns dict(E, m2, ls, alive, offset)."""
+    t = bp.step_table()
+    coeffs, ov, ys, offset = bp.step_parameters(f, yfix)
+    n = len(t['kind'])
+    dt = np.complex128 if storage else np.complex64
+    E = buffer if buffer is not None else np.zeros(1 << bp.inplace_maxslots, dt)
+    m2 = np.zeros(n, np.float64)
+    arrs = {k: np.ascontiguousarray(t[k]) for k in ('kind', 'K', 'Ka', 'pivot', 'pivot_a', 'beta', 'in_mask')}
+    lib().oracle_contract(int(storage), n, _p(arrs['kind'], C.c_int32), _p(arrs['K'], C.c_uint64),
+                          _p(arrs['Ka'], C.c_uint64), _p(arrs['pivot'], C.c_int32), _p(arrs['pivot_a'], C.c_int32),
+                          _p(arrs['beta'], C.c_uint64), _p(arrs['in_mask'], C.c_uint64),
+                          _p(np.ascontiguousarray(coeffs), C.c_double), _p(np.ascontiguousarray(ov), C.c_int32),
+                          _p(np.ascontiguousarray(ys), C.c_int32), E.ctypes.data_as(C.c_void_p), _p(m2, C.c_double))
+    ls, alive = frexp_scale_chain(m2)
+    return {'E': E, 'm2': m2, 'ls': ls, 'alive': alive, 'offset': offset}
+
+
+def terminal_env(bp, E):
+    idx = np.zeros(1 << bp.terminal_rank, np.uint64)
+    ids = np.arange(1 << bp.terminal_rank, dtype=np.uint64)
+    for j, p in enumerate(bp.terminal_slots):
+        idx |= ((ids >> np.uint64(j)) & np.uint64(1)) << np.uint64(p)
+    return E[idx.astype(np.int64)]
+
+
+def block(bp, x_bits, w=None, storage=1, part=0, parts=1):
+    """Complex128 block sqrt(D) psi[prefix, :] by the oracle path (conditional_block_gpu.py:128-177)."""
+    k = bp.k
+    total = np.zeros(1 << k, np.complex128)
+    tv = total.view(np.float64)
+    smap = np.ascontiguousarray(bp.syndrome_map())
+    f_full = bp.local_vectors(x_bits, w=w)
+    dt = np.complex128 if storage else np.complex64
+    E = np.zeros(1 << bp.inplace_maxslots, dt)
+    for _, coef, f, yfix, freeflip in bp.slice_assignments(f_full, part, parts):
+        if f is None:
+            continue
+        r = contract_inplace(bp, f, yfix, storage, buffer=E)
+        if not r['alive']:
+            continue
+        c = coef * bp.slice_scale(r['ls'])
+        xm = (r['offset'] >> bp.stop) ^ freeflip
+        lib().oracle_accumulate(int(storage), E.ctypes.data_as(C.c_void_p), C.c_uint64(bp.terminal_mask()), k,
+                                _p(smap, C.c_uint64), C.c_uint64(xm), c.real, c.imag, _p(tv, C.c_double))
+    ph = np.ascontiguousarray(bp.phase())
+    U = np.ascontiguousarray(bp.open_unitaries().reshape(-1, 4).view(np.float64))
+    lib().oracle_transform(k, _p(tv, C.c_double), _p(ph, C.c_double), _p(U, C.c_double))
+    return total
+
+
+def draw(v, u):
+    """Primary CDF draw; returns dict(A_index, selected_scaled_weight, scaled_mass_cdf, cdf_lower_numpy,
+    cdf_upper_numpy) or raises ArithmeticError."""
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    n = len(v)
+    cum = np.zeros(n, np.float64)
+    out = np.zeros(5, np.float64)
+    rc = lib().oracle_draw(_p(v.view(np.float64), C.c_double), n, float(u), _p(cum, C.c_double), _p(out, C.c_double))
+    if rc:
+        raise ArithmeticError(f'oracle draw failed ({rc})')
+    return {'A_index': int(out[0]), 'selected_scaled_weight': out[1], 'scaled_mass_cdf': out[2],
+            'cdf_lower_numpy': out[3], 'cdf_upper_numpy': out[4]}
+
+
+# ---- mathematical ground truth ---------------------------------------------------------------
+_GATES = {
+    'H': np.array([[1, 1], [1, -1]], np.complex128) / math.sqrt(2),
+    'S': np.diag([1, 1j]).astype(np.complex128),
+    'T': np.diag([1, np.exp(1j * math.pi / 4)]).astype(np.complex128),
+}
+
+
+def statevector(ops, n):
+    """<x|C|0> for every x (bit q of the index is qubit q)."""
+    psi = np.zeros((2,) * n, np.complex128)
+    psi[(0,) * n] = 1
+    for op in ops:
+        if op[0] in ('c1', 'T'):
+            q = op[1]
+            g = _GATES['T'] if op[0] == 'T' else _GATES[op[2]]
+            ax = n - 1 - q
+            psi = np.moveaxis(np.tensordot(g, psi, axes=([1], [ax])), 0, ax)
+        elif op[0] == 'cz':
+            a, b = n - 1 - op[1], n - 1 - op[2]
+            idx = [slice(None)] * n
+            idx[a] = 1
+            idx[b] = 1
+            psi[tuple(idx)] *= -1
+        else:
+            raise ValueError(op)
+    return psi.reshape(-1)
+
+
+def graph_amplitude(A, U, bras):
+    """Brute-force <v|U|G> = 2^{-N/2} sum_y prod f_k(y_k) (-1)^{sum_E y_a y_b}, f_k = bra_k U_k."""
+    N = len(A)
+    f = np.array([bras[q] @ U[q] for q in range(N)])
+    total = 0j
+    iu = np.triu_indices(N, 1)
+    for yint in range(1 << N):
+        y = np.array([(yint >> q) & 1 for q in range(N)])
+        w = np.prod(f[np.arange(N), y])
+        if w == 0:
+            continue
+        s = int((np.outer(y, y)[iu] * A[iu]).sum()) & 1
+        total += w * (-1 if s else 1)
+    return total * 2 ** (-N / 2)
+
+
+def gather_amplitude(N, edges, order, fixed, f, yfix):
+    """Full-scan amplitude (up to 2^{-N/2}) by the gather executor: returns (value, logscale)
+    (scan_plan.py:161-195), using this repository's make_plan for the plan."""
+    import sys
+    sys.path.insert(0, str(HERE.parent / 'dcs245-classical-sampling_amd'))
+    from dcs_amd.scan_plan import byte_tables, future_masks, kernel_combos, make_plan
+    steps, _ = make_plan(N, edges, order, fixed)
+    fut = future_masks(N, edges, order)
+    E = np.ones(1, np.complex128)
+    o = 0
+    ls = 0.
+    for j, st in enumerate(steps):
+        f0, f1 = complex(f[st.v][0]), complex(f[st.v][1])
+        ov = (o >> j) & 1
+        o &= ~(1 << j)
+        y = 0
+        if not st.free:
+            y = int(yfix[st.v])
+            if y:
+                o ^= fut[j]
+        T = byte_tables(st.Lcols, st.r_out)
+        cp = np.arange(1 << st.r_out, dtype=np.uint64)
+        p0 = np.zeros_like(cp)
+        for b in range(T.shape[0]):
+            p0 ^= T[b][((cp >> np.uint64(8 * b)) & np.uint64(255)).astype(np.int64)]
+        mask = np.uint64((1 << st.r_in) - 1)
+        acc = np.zeros(1 << st.r_out, np.complex128)
+        for kc in kernel_combos(st.kernel):
+            p = p0 ^ np.uint64(kc)
+            c = p & mask
+            par = np.array([bin(int(v) & st.beta).count('1') & 1 for v in c], np.int64) ^ ov
+            val = E[c.astype(np.int64)]
+            if st.free:
+                yb = ((p >> np.uint64(st.r_in)) & np.uint64(1)).astype(np.int64)
+                sg = 1 - 2 * (yb & par)
+                acc += val * np.where(yb == 1, f1, f0) * sg
+            else:
+                sg = 1 - 2 * (y & par)
+                acc += val * (f1 if y else f0) * sg
+        E = acc
+        m = float(np.max(np.abs(E))) if E.size else 0.
+        if m > 0:
+            E = E / m
+            ls += math.log(m)
+    return complex(E[0]), ls
