@@ -1,0 +1,216 @@
+"""Benchmark: samples/s of the 64-qubit, 2^20-block conditional sampler on one MI355X.
+
+Workload: the frozen 378-vertex injected graph of the 64-qubit, depth-73, 314-T ring circuit (IBM
+QAT #245), 20 free output bits (2^20 complex128 block per sample), 44-bit prefixes and draw uniforms
+replayed from the frozen PCG64 protocol (its first samples).  One step = one sample:
+every slice contraction, the coherent slice sum, the Walsh-Hadamard / phase / Clifford tail and the
+device CDF draw.  Inputs (graph, plan, frontier buffer) are resident before the timed region.
+
+Fast path (value): complex64 storage, plan inplace_k2_fp32 (4 slices, rmax 34, 128 GiB frontier),
+fused schedule (LDS-resident runs + coset groups).  Plain path (reported beside it): complex128
+storage, plan inplace_k3_fp64 (8 slices, rmax 33), one launch per step and a host read of every
+maximum, one prefix at a time -- the reference method.
+
+The line also carries the roofline of the dominant kernel (k_coset, HBM-bound) measured with HIP
+events on the library's stream, and a CPU baseline: the C oracle (OpenMP) on a bounded sample of the
+same workload (the first steps of one slice, widths <= 2^28), extrapolated by bytes.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / 'dcs245-classical-sampling_amd'))
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12
+# Generation seed of the frozen 64-qubit protocol (its 1,389 prefixes and uniforms are replayed).
+PROTOCOL_SEED = 10636914113228816500957915140926462448
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def prefixes(count):
+    from dcs_amd.sampling_protocol import make_protocol
+    from dcs_amd.workloads import load_ibm64
+    w = load_ibm64()
+    p = make_protocol(PROTOCOL_SEED, count, 64, w['open_vertices'], 1.0)
+    return [(np.array([int(c) for c in r['input_data_bits']], np.uint8), r['draw_uniform']) for r in p['records']]
+
+
+def fused_bytes_per_sample(engine, elem):
+    """Algorithmic HBM bytes of the fused schedule per sample, per segment kind."""
+    bp = engine.bp
+    st = bp.inplace_steps
+    out = {0: 0, 1: 0, 2: 0}
+    for kind, first, count, _ in engine.schedule():
+        last = first + count - 1
+        out[kind] += ((1 << len(st[first].active_in)) + (1 << len(st[last].active_out))) * elem
+    nslices = 1 << len(bp.slices)
+    return {k: v * nslices for k, v in out.items()}
+
+
+def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False):
+    from dcs_amd.conditional_block import InplaceConditionalBlockGPU
+    from dcs_amd.workloads import load_ibm64
+    w = load_ibm64()
+    e = InplaceConditionalBlockGPU(w['graph'], w['unitaries'], 64, w['order'], w['plans'][plan_name]['operations'],
+                                   w['open_vertices'])
+    t0 = time.perf_counter()
+    e.alloc(dtype=dtype, schedule=schedule)
+    setup = time.perf_counter() - t0
+    work = prefixes(warmup + nsamples)
+    for x, u in work[:warmup]:
+        vec, _ = e.block(x)
+        vec.draw(u)
+    e.ctx.synchronize()
+    lat = []
+    draws = []
+    t0 = time.perf_counter()
+    for x, u in work[warmup:]:
+        s = time.perf_counter()
+        vec, meta = e.block(x)
+        d = vec.draw(u)
+        lat.append(time.perf_counter() - s)
+        draws.append(d['A_index'])
+    e.ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = None
+    if profile:
+        # one more sample outside the timed region with HIP events around every launch
+        e._plan.profile(True)
+        x, u = work[-1]
+        vec, _ = e.block(x)
+        vec.draw(u)
+        prof = e._plan.profile(False)
+    info = {'engine': e, 'elapsed': elapsed, 'lat': lat, 'setup': setup, 'draws': draws, 'profile': prof}
+    return info
+
+
+def cpu_baseline(max_width=28):
+    """C oracle, complex64 storage, on the first steps of one k2 slice whose live width is <= max_width."""
+    from oracle import oracle
+    from dcs_amd.block_plan import BlockPlan
+    from dcs_amd.workloads import load_ibm64
+    import ctypes as C
+    w = load_ibm64()
+    bp = BlockPlan(w['graph'], w['unitaries'], 64, w['order'], w['plans']['inplace_k2_fp32']['operations'],
+                   w['open_vertices'])
+    x, _ = prefixes(1)[0]
+    f_full = bp.local_vectors(x)
+    _, coef, f, yfix, _ = next(bp.slice_assignments(f_full))
+    t = bp.step_table()
+    coeffs, ov, ys, _ = bp.step_parameters(f, yfix)
+    m = 0
+    while m < len(bp.inplace_steps) and max(len(bp.inplace_steps[m].active_in), len(bp.inplace_steps[m].active_out)) <= max_width:
+        m += 1
+    nbytes = sum(((1 << len(s.active_in)) + (1 << len(s.active_out))) * 8 for s in bp.inplace_steps[:m])
+    E = np.zeros(1 << max_width, np.complex64)
+    m2 = np.zeros(m, np.float64)
+    arr = {k: np.ascontiguousarray(t[k][:m]) for k in ('kind', 'K', 'Ka', 'pivot', 'pivot_a', 'beta', 'in_mask')}
+    P = lambda a, ct: a.ctypes.data_as(C.POINTER(ct))
+    L = oracle.lib()
+    t0 = time.perf_counter()
+    L.oracle_contract(0, m, P(arr['kind'], C.c_int32), P(arr['K'], C.c_uint64), P(arr['Ka'], C.c_uint64),
+                      P(arr['pivot'], C.c_int32), P(arr['pivot_a'], C.c_int32), P(arr['beta'], C.c_uint64),
+                      P(arr['in_mask'], C.c_uint64), P(np.ascontiguousarray(coeffs[:m]), C.c_double),
+                      P(np.ascontiguousarray(ov[:m]), C.c_int32), P(np.ascontiguousarray(ys[:m]), C.c_int32),
+                      E.ctypes.data_as(C.c_void_p), P(m2, C.c_double))
+    dt = time.perf_counter() - t0
+    per_sample = bp.bytes_per_slice(8) * (1 << len(bp.slices))
+    rate = nbytes / dt
+    return {'value': rate / per_sample, 'unit': 'samples/s', 'cores': int(L.oracle_threads()), 'kind': 'port',
+            'sample': f'C oracle (OpenMP, complex64 storage, FP64 arithmetic) over the first {m} steps of one '
+                      f'k2 slice (live width <= 2^{max_width}), {nbytes / 1e9:.1f} GB in {dt:.1f} s '
+                      f'= {rate / 1e9:.2f} GB/s, extrapolated by the {per_sample:.4g} algorithmic bytes of a sample',
+            'seconds': dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=4)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--plain-samples', type=int, default=1)
+    ap.add_argument('--schedule', default='fused')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-plain', action='store_true')
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({'error': 'this benchmark runs on exactly one GPU; the sampler shards by sample '
+                          'and needs no collectives', 'n_gpus': a.gpus}))
+        sys.exit(2)
+    from dcs_amd import _native
+    dev = _native.device_info(0)
+    log('device', dev)
+
+    log('fast path: complex64, k2 plan, schedule', a.schedule)
+    fast = run_path('inplace_k2_fp32', np.complex64, a.schedule, a.steps, a.warmup, profile=True)
+    eng = fast['engine']
+    ms, launches = fast['profile']
+    sps = a.steps / fast['elapsed']
+    bp = eng.bp
+    unfused_bytes = bp.bytes_per_slice(8) * (1 << len(bp.slices))
+    fb = fused_bytes_per_sample(eng, 8)
+    coset_ms = ms[2]
+    coset_bytes = fb[2]
+    achieved = coset_bytes / (coset_ms / 1e3) if coset_ms > 0 else None
+    log(f'fast: {sps:.4f} samples/s, kernel ms/sample: step {ms[0]:.1f} resident {ms[1]:.1f} '
+        f'coset {coset_ms:.1f} block {ms[3]:.1f} draw {ms[4]:.1f}; launches {launches.tolist()}')
+    eng.close()
+    del eng, fast['engine']
+
+    plain = None
+    if not a.no_plain and a.plain_samples > 0:
+        log('plain path: complex128, k3 plan, per-step launches with host maxima')
+        p = run_path('inplace_k3_fp64', np.complex128, 'steps_host', a.plain_samples, 0)
+        pb = p['engine'].bp
+        pbytes = pb.bytes_per_slice(16) * (1 << len(pb.slices))
+        plain = {'samples_per_s': a.plain_samples / p['elapsed'], 'ms_per_sample': 1e3 * p['elapsed'] / a.plain_samples,
+                 'algorithmic_bytes_per_sample': pbytes,
+                 'hbm_GBps': pbytes * a.plain_samples / p['elapsed'] / 1e9,
+                 'roofline_frac': pbytes * a.plain_samples / p['elapsed'] / HBM_PEAK}
+        log('plain', plain)
+        p['engine'].close()
+
+    cpu = None
+    if not a.no_cpu:
+        log('cpu baseline ...')
+        cpu = cpu_baseline()
+        log('cpu', cpu)
+
+    lat = np.array(fast['lat']) * 1e3
+    line = {
+        'metric': 'samples/s @64q 2^20 block: batched c64 vs plain c128, HBM GB/s; p50/p99 ms',
+        'value': sps, 'unit': 'samples/s', 'n_gpus': 1, 'steps': a.steps, 'warmup': a.warmup,
+        'ms_per_sample': 1e3 * fast['elapsed'] / a.steps, 'ms_per_step': 1e3 * fast['elapsed'] / a.steps,
+        'p50_ms': float(np.percentile(lat, 50)), 'p99_ms': float(np.percentile(lat, 99)),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'c64 storage / f64 arithmetic', 'data': 'frozen IBM QAT #245 graph, protocol-replayed prefixes',
+        'config': {'workload': 'IBM QAT #245 64q/d73/314T injected graph (N=378), plan inplace_k2_fp32 '
+                               '(4 parity slices, rmax 34, 128 GiB c64 frontier), 2^20 block, 44-bit prefixes',
+                   'schedule': a.schedule, 'block': 1 << 20},
+        'effective_hbm_GBps': unfused_bytes * sps / 1e9,
+        'effective_roofline_frac': unfused_bytes * sps / HBM_PEAK,
+        'algorithmic_bytes_per_sample': {'unfused': unfused_bytes, 'fused': sum(fb.values())},
+        'plain_c128': plain,
+        'speedup_vs_plain_c128': (sps / plain['samples_per_s']) if plain else None,
+        'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9 if achieved else None, 'peak': HBM_PEAK / 1e9,
+                     'unit': 'GB/s', 'frac': (achieved / HBM_PEAK) if achieved else None, 'traffic': None,
+                     'kernel': 'k_coset', 'bytes_per_sample': coset_bytes, 'ms_per_sample': coset_ms},
+        'cpu_baseline': cpu,
+        'device': {k: dev[k] for k in ('device_name', 'arch', 'compute_units', 'total_bytes')},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

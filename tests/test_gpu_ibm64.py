@@ -1,0 +1,69 @@
+"""The frozen 64-qubit QAT #245 graph on the device against the reference's run records.
+
+* complex128, plan inplace_k3_fp64 (8 slices, rmax 33, 128 GiB frontier): the amplitude at the drawn
+  index agrees with fullscore/analysis/scores.jsonl to 1e-10 relative (the reference's own c128
+  reruns agree to 2.2e-16; the difference here is the FMA placement of the CUDA build), and the block
+  mass with the audit record to 1e-10 relative.
+* complex64, plan inplace_k2_fp32 (4 slices, rmax 34): the draw index equals the recorded r1 draw
+  unless the recorded CDF margin is below 1e-7 (a reported tie); amplitudes at the drawn index agree
+  with the c128 scores to 1e-4 relative and block masses to 1e-5."""
+import numpy as np
+import pytest
+
+from dcs_amd.conditional_block import InplaceConditionalBlockGPU
+
+pytestmark = pytest.mark.gpu
+
+TIE = 1e-7
+
+
+def make(ibm64, plan, dtype, **kw):
+    e = InplaceConditionalBlockGPU(ibm64['graph'], ibm64['unitaries'], 64, ibm64['order'],
+                                   ibm64['plans'][plan]['operations'], ibm64['open_vertices'])
+    e.alloc(dtype=dtype, **kw)
+    assert e.inplace_rmax == ibm64['plans'][plan]['rmax']
+    return e
+
+
+def bits(s):
+    return np.array([int(c) for c in s], np.uint8)
+
+
+def test_ibm64_c128_k3_matches_scores(ibm64, records):
+    print('case ibm64 c128 k3 fused', flush=True)
+    e = make(ibm64, 'inplace_k3_fp64', np.complex128, schedule='fused')
+    audited = sorted(int(k) for k in records['audits'])[:1]
+    for sid in [0] + audited:
+        s = records['samples'][str(sid)]
+        vec, meta = e.block(bits(s['bits']))
+        v = np.asarray(vec)
+        A, re, im, _ = records['scores'][str(sid)]
+        want = complex(re, im)
+        print(f'sample {sid}: {meta["seconds"]:.2f} s, amp {v[A]} vs {want}', flush=True)
+        assert abs(v[A] - want) <= 1e-10 * abs(want)
+        assert meta['slices_evaluated'] == 8 and meta['rmax'] == 33 and meta['work_per_slice'] == 315122088218
+        if str(sid) in records['audits']:
+            mass = float(np.vdot(v, v).real)
+            assert abs(mass - records['audits'][str(sid)]['ref_mass']) <= 1e-10
+    e.close()
+
+
+def test_ibm64_c64_k2_draws_match_r1(ibm64, records):
+    print('case ibm64 c64 k2 fused', flush=True)
+    e = make(ibm64, 'inplace_k2_fp32', np.complex64, schedule='fused')
+    ties = 0
+    for sid in range(3):
+        s = records['samples'][str(sid)]
+        vec, meta = e.block(bits(s['bits']))
+        d = vec.draw(s['u'])
+        v = np.asarray(vec)
+        A, re, im, _ = records['scores'][str(sid)]
+        print(f'sample {sid}: {meta["seconds"]:.2f} s, index {d["A_index"]} vs {s["A_index"]}, '
+              f'mass {d["scaled_mass_cdf"]} vs {s["mass"]}', flush=True)
+        if s['margin'] < TIE:
+            ties += 1
+        else:
+            assert d['A_index'] == s['A_index']
+        assert abs(v[A] - complex(re, im)) <= 1e-4 * abs(complex(re, im))
+        assert abs(d['scaled_mass_cdf'] - s['mass']) <= 1e-5
+    e.close()

@@ -1,0 +1,185 @@
+"""HIP path against the oracle, through the C ABI, on seeded ring circuits.
+
+Tolerances: complex128 per-step schedule == oracle bit for bit; fused schedules round to storage
+precision after every step exactly like the per-step one, so they differ only through exp(ls) of the
+slice coefficient (<= 1e-13 relative); complex64 storage vs the complex128 oracle <= 1e-5 relative
+to the block's largest amplitude (BASELINE target)."""
+import numpy as np
+import pytest
+
+from dcs_amd.block_plan import BlockPlan
+from dcs_amd.conditional_block import InplaceConditionalBlockGPU
+from dcs_amd.sampling_protocol import cdf_primary, draw_checked
+from dcs_amd.stabilizer import synthetic_instance
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+@pytest.fixture(scope='module')
+def c1():
+    inst = synthetic_instance(16, 12, 20, seed=0)
+    args = (inst['graph'], inst['unitaries'], 16, inst['natural_order'], [], list(range(16)))
+    bp = BlockPlan(*args)
+    x = np.zeros(16, np.uint8)
+    return {'args': args, 'x': x, 'ref128': oracle.block(bp, x), 'ref64': oracle.block(bp, x, storage=0),
+            'psi': oracle.statevector(inst['ops'], 16)}
+
+
+def engine(args, **kw):
+    e = InplaceConditionalBlockGPU(*args)
+    e.alloc(**kw)
+    return e
+
+
+def test_c1_c128_steps_bit_exact(c1):
+    print('case c1 c128 steps', flush=True)
+    e = engine(c1['args'], dtype=np.complex128, schedule='steps')
+    v, meta = e.block(c1['x'])
+    assert np.array_equal(np.asarray(v), c1['ref128'])
+    assert meta['prefix_dtype'] == 'complex128' and meta['slices_evaluated'] == 1
+    e.close()
+
+
+def test_c1_c128_host_chain_bit_exact(c1):
+    print('case c1 c128 steps_host', flush=True)
+    e = engine(c1['args'], dtype=np.complex128, schedule='steps_host')
+    v, _ = e.block(c1['x'])
+    assert np.array_equal(np.asarray(v), c1['ref128'])
+    e.close()
+
+
+def test_c1_c64_steps_bit_exact(c1):
+    print('case c1 c64 steps', flush=True)
+    e = engine(c1['args'], dtype=np.complex64, schedule='steps')
+    v, _ = e.block(c1['x'])
+    assert np.array_equal(np.asarray(v), c1['ref64'])
+    assert rel(v, c1['ref128']) < 1e-5
+    e.close()
+
+
+@pytest.mark.parametrize('rb,dim,tl', [(12, 6, 12), (6, 6, 12), (6, 8, 11), (0, 4, 10), (6, 1, 12)])
+def test_c1_fused_matches_steps(c1, rb, dim, tl):
+    print(f'case c1 fused rb={rb} dim={dim} tl={tl}', flush=True)
+    for dt, ref, tol in ((np.complex128, c1['ref128'], 1e-13), (np.complex64, c1['ref64'], 1e-13)):
+        e = engine(c1['args'], dtype=dt, schedule='fused', resident_bits=rb, fuse_max_dim=dim, tile_log2=tl)
+        v, _ = e.block(c1['x'])
+        assert rel(v, ref) < tol, (dt, rel(v, ref))
+        e.close()
+
+
+def test_c1_matches_statevector_and_64_draws_seed0(c1):
+    """C1: all 2^16 amplitudes agree with the dense state vector; 64 draws from seed 0 are identical
+    between device draw, host draw_checked and the oracle."""
+    print('case c1 draws', flush=True)
+    e = engine(c1['args'], dtype=np.complex128, schedule='fused')
+    vec, _ = e.block(c1['x'])
+    v = np.asarray(vec)
+    psi = c1['psi']
+    i = int(np.argmax(np.abs(psi)))
+    assert rel(v, (v[i] / psi[i]) * psi) < 1e-12
+    assert abs(np.vdot(v, v).real - 1) < 1e-12
+    rng = np.random.Generator(np.random.PCG64(0))
+    for u in rng.random(64):
+        d = vec.draw(u)
+        h = draw_checked(v, u)
+        o = oracle.draw(v, u)
+        assert d['A_index'] == h['A_index'] == o['A_index']
+        assert d['cdf_lower_numpy'] == h['cdf_lower_numpy'] and d['cdf_upper_numpy'] == h['cdf_upper_numpy']
+        assert d['scaled_mass_cdf'] == h['scaled_mass_cdf']
+    e.close()
+
+
+def test_prefix_blocks_and_lanes():
+    """12 qubits, 6 open vertices, several prefixes; lanes=3 batching gives the same blocks."""
+    print('case prefix lanes', flush=True)
+    inst = synthetic_instance(12, 10, 14, seed=1)
+    opn = [9, 2, 7, 11, 4, 0]
+    args = (inst['graph'], inst['unitaries'], 12, inst['natural_order'], [], opn)
+    bp = BlockPlan(*args)
+    rng = np.random.default_rng(5)
+    xs = [rng.integers(0, 2, 12, dtype=np.uint8) for _ in range(4)]
+    refs = [oracle.block(bp, x) for x in xs]
+    e1 = engine(args, dtype=np.complex128, schedule='steps')
+    for x, r in zip(xs, refs):
+        assert np.array_equal(np.asarray(e1.block(x)[0]), r)
+    e1.close()
+    e3 = engine(args, dtype=np.complex64, schedule='fused', lanes=3, resident_bits=6)
+    for x, r in zip(xs, refs):
+        assert rel(e3.block(x)[0], r) < 1e-5
+    e3.close()
+
+
+def test_parity_sliced_ring_circuit():
+    """Parity slices on reference vertices (Y, X and Z operations) against the unsliced oracle."""
+    print('case parity slices', flush=True)
+    inst = synthetic_instance(10, 10, 12, seed=2)
+    n = 10
+    opn = [0, 3, 6, 8]
+    refs = [v for v in range(n, inst['N']) if inst['graph'][v].any()]
+    nb = [int(u) for u in np.flatnonzero(inst['graph'][refs[2]])]
+    ops = [('Y', refs[0], None), ('Z', refs[1], None), ('X', refs[2], nb[0])]
+    x = np.array([1, 0, 0, 1, 1, 0, 0, 1, 0, 1], np.uint8)
+    full = oracle.block(BlockPlan(inst['graph'], inst['unitaries'], n, inst['natural_order'], [], opn), x)
+    args = (inst['graph'], inst['unitaries'], n, inst['natural_order'], ops, opn)
+    sl = oracle.block(BlockPlan(*args), x)
+    # parity slicing changes local Cliffords by exact phases: compare up to one global phase
+    i = int(np.argmax(np.abs(full)))
+    assert rel(sl, (sl[i] / full[i]) * full) < 1e-12
+    e = engine(args, dtype=np.complex128, schedule='steps', lanes=2)
+    v, meta = e.block(x)
+    assert np.array_equal(np.asarray(v), sl)
+    assert meta['slices_evaluated'] == 8
+    e.close()
+
+
+def test_nonfinite_prefix_raises(c1):
+    print('case nonfinite', flush=True)
+    e = engine(c1['args'], dtype=np.complex64, schedule='fused')
+    w = {q: np.array([np.nan, 1.0]) for q in range(16, 36)}
+    with pytest.raises(ArithmeticError):
+        e.block(c1['x'], w=w)
+    e.close()
+
+
+def test_zero_block_draw_raises(c1):
+    print('case zero block', flush=True)
+    e = engine(c1['args'], dtype=np.complex128, schedule='steps')
+    w = {q: np.array([0.0, 0.0]) for q in range(16, 36)}
+    vec, _ = e.block(c1['x'], w=w)
+    assert not np.any(np.asarray(vec))
+    with pytest.raises(ArithmeticError):
+        vec.draw(0.5)
+    e.close()
+
+
+def test_admission_refuses_oversized_frontier():
+    print('case admission', flush=True)
+    from dcs_amd import _native
+    ctx = _native.Context(0)
+    t = {'kind': [0], 'K': [0], 'Ka': [0], 'pivot': [39], 'pivot_a': [-1], 'beta': [0], 'in_mask': [0],
+         'out_mask': [1 << 39]}
+    with pytest.raises(MemoryError):
+        _native.Plan(ctx, t, 40, 1, 2, 1, 1 << 39, np.array([0, 1], np.uint64), np.ones(2),
+                     np.eye(2)[None].astype(complex))
+    ctx.close()
+
+
+def test_device_draw_random_vectors():
+    """Device sequential-cumsum draw equals numpy's draw on random blocks (index and both bounds)."""
+    print('case device draw random', flush=True)
+    inst = synthetic_instance(14, 10, 16, seed=9)
+    e = engine((inst['graph'], inst['unitaries'], 14, inst['natural_order'], [], list(range(14))),
+               dtype=np.complex64, schedule='fused')
+    vec, _ = e.block(np.zeros(14, np.uint8))
+    v = np.asarray(vec)
+    for u in np.random.default_rng(1).random(200):
+        d = vec.draw(u)
+        idx, w, cum, tot = cdf_primary(v, u)
+        assert d['A_index'] == idx and d['scaled_mass_cdf'] == tot
+        assert d['cdf_upper_numpy'] == cum[idx] and d['cdf_lower_numpy'] == (cum[idx - 1] if idx else 0.)
+    e.close()
