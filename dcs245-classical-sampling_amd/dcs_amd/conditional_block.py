@@ -90,13 +90,15 @@ class InplaceConditionalBlockGPU:
         self._plan = None
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='fused',
-              fuse_max_dim=6, resident_bits=12, tile_log2=12, blocks=1):
+              fuse_max_dim=6, resident_bits=12, tile_log2=12, blocks=1, exact_rounding=False):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
         lanes      frontiers held at once; slices of one block are contracted `lanes` at a time
         schedule   'fused' (resident runs + coset groups), 'steps' (one launch per step, scale chain
                    on the device) or 'steps_host' (one launch per step, host reads every maximum)
+        exact_rounding  fused schedule only: round to storage precision after every step (mantissas equal
+                   the per-step schedule) instead of once per register pass of up to three steps
         The legacy fp64_kernel_path argument is accepted and ignored, as in the reference."""
         dt = np.dtype(dtype)
         if dt not in (np.dtype(np.complex64), np.dtype(np.complex128)):
@@ -116,7 +118,7 @@ class InplaceConditionalBlockGPU:
         self._plan = _native.Plan(self.ctx, bp.step_table(), bp.inplace_maxslots, storage, self.lanes, bp.k,
                                   bp.terminal_mask(), bp.syndrome_map(), bp.phase(), bp.open_unitaries(),
                                   blocks=blocks, fuse_max_dim=fuse_max_dim, resident_bits=resident_bits,
-                                  tile_log2=tile_log2)
+                                  tile_log2=tile_log2, exact_rounding=exact_rounding)
         self.buf = True
 
     # ---- reference mixin hooks -------------------------------------------------------------

@@ -31,6 +31,7 @@
 
 #include "../../../include/dcs_amd.h"
 #include "dcs_kernels.hpp"
+#include "coset_pass.hpp"
 
 namespace {
 
@@ -116,18 +117,12 @@ struct StepDev {
 };
 
 struct GroupDev {
-    int32_t d, wbits, lane_identity, nsteps;
-    int32_t first_gstep, addrv_offset;
+    int32_t d, wbits, lane_identity, npasses;
+    int32_t first_pass, addrv_offset;
     uint32_t dead_in_t, dead_out_t;
     u64 rep_low_mask;
     u64 ntiles;
     Runs high;
-};
-
-struct GStepDev {
-    u64 beta;
-    int32_t kind, step, tp, tpa;
-    uint32_t tk, tka, bvec, beta_lane, inactive_t, pad;
 };
 
 struct DrawOut {
@@ -328,89 +323,12 @@ __global__ void __launch_bounds__(kThreads) k_resident(S *__restrict__ base, u64
 // ------------------------------------------------------------------------------------------------
 // k_coset: fused group.  Tile layout: element (t, l) at LDS index t*W + l, t in [0, 2^d) the
 // coordinate in the RREF basis of V (address = c0(l) ^ addrv[t]), l the lane coordinate.
-template <class S>
-__device__ __forceinline__ void coset_step(S *tile, const GStepDev &st, const Dyn &dd, int d, int wb, int ptile) {
-    const cd f0{dd.f0r, dd.f0i}, f1{dd.f1r, dd.f1i};
-    const int y = dd.y;
-    const uint32_t W = 1u << wb;
-    const int kind = st.kind;
-    const int nfix = (kind == 2 || kind == 4) ? 0 : (kind == 6 ? 2 : 1);
-    const uint32_t norb = (1u << (d - nfix)) << wb;
-    const int lo = min(st.tp, st.tpa), hi = max(st.tp, st.tpa);
-    for (uint32_t o = threadIdx.x; o < norb; o += blockDim.x) {
-        const uint32_t l = o & (W - 1);
-        const uint32_t tt = o >> wb;
-        uint32_t t;
-        if (nfix == 0) t = tt;
-        else if (nfix == 1) t = insert0(tt, st.tp);
-        else t = insert0(insert0(tt, lo), hi);
-        if (t & st.inactive_t) continue;
-        const int pl = ptile ^ (__popc(l & st.beta_lane) & 1);
-#define SB(tv) (pl ^ (__popc((tv) & st.bvec) & 1))
-        const uint32_t ic = t * W + l;
-        const cd va = Store<S>::load(tile[ic]);
-        const int sc = SB(t);
-        switch (kind) {
-            case 0: {
-                cd o0, o1;
-                orb_grow(va, f0, f1, sc, o0, o1);
-                tile[ic] = Store<S>::round(o0);
-                tile[(t | (1u << st.tp)) * W + l] = Store<S>::round(o1);
-            } break;
-            case 1: {
-                const uint32_t u = t ^ st.tka;
-                const uint32_t iu = u * W + l;
-                const cd vb = Store<S>::load(tile[iu]);
-                cd o0, o1;
-                orb_mix_pair(va, vb, f0, f1, sc, SB(u), o0, o1);
-                tile[ic] = Store<S>::round(o0);
-                tile[iu] = Store<S>::round(o1);
-            } break;
-            case 2:
-                tile[ic] = Store<S>::round(orb_mix_diag(va, f0, f1, sc));
-                break;
-            case 3: {
-                const uint32_t u = t ^ st.tk;
-                const uint32_t iu = u * W + l;
-                const cd vb = Store<S>::load(tile[iu]);
-                cd o0, o1;
-                orb_replace(va, vb, f0, f1, sc, SB(u), o0, o1);
-                tile[ic] = Store<S>::round(o0);
-                tile[iu] = Store<S>::round(o1);
-            } break;
-            case 4:
-                tile[ic] = Store<S>::round(orb_fixed(va, f0, f1, sc, y));
-                break;
-            case 5: {
-                const uint32_t u = t ^ st.tk;
-                const cd vb = Store<S>::load(tile[u * W + l]);
-                tile[ic] = Store<S>::round(orb_fixed_shrink(va, vb, f0, f1, sc, SB(u), y));
-            } break;
-            case 6: {
-                const uint32_t c1 = t ^ st.tk, dd_ = t ^ st.tka, d1 = dd_ ^ st.tk;
-                const cd vb = Store<S>::load(tile[c1 * W + l]);
-                const cd ve = Store<S>::load(tile[dd_ * W + l]);
-                const cd vg = Store<S>::load(tile[d1 * W + l]);
-                cd o0, o1;
-                orb_shrink_quad(va, vb, ve, vg, f0, f1, sc, SB(c1), SB(dd_), SB(d1), o0, o1);
-                tile[ic] = Store<S>::round(o0);
-                tile[dd_ * W + l] = Store<S>::round(o1);
-            } break;
-            default: {
-                const uint32_t u = t ^ st.tk;
-                const cd vb = Store<S>::load(tile[u * W + l]);
-                tile[ic] = Store<S>::round(orb_shrink_pair(va, vb, f0, f1, sc, SB(u)));
-            } break;
-        }
-#undef SB
-    }
-}
-
-template <class S, int TL>
+template <class S, int TL, bool EXACT>
 __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 lane_stride,
                                                     const int *__restrict__ lane_ids,
                                                     const GroupDev *__restrict__ groups, int gi,
-                                                    const GStepDev *__restrict__ gsteps,
+                                                    const PassDev *__restrict__ passes,
+                                                    const PStepDev *__restrict__ psteps,
                                                     const u64 *__restrict__ addrv_all,
                                                     const Dyn *__restrict__ dyn, int nsteps,
                                                     u64 *__restrict__ dmax, int nunits, int unit) {
@@ -419,6 +337,7 @@ __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 la
     const GroupDev &g = groups[gi];
     const int li = blockIdx.y;
     S *__restrict__ E = base + (u64)lane_ids[li] * lane_stride;
+    const Dyn *dyn_lane = dyn + (u64)li * nsteps;
     const int d = g.d, wb = g.wbits;
     const uint32_t W = 1u << wb;
     const uint32_t n = (1u << d) << wb;
@@ -439,11 +358,15 @@ __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 la
             tile[e] = v;
         }
         __syncthreads();
-        for (int s = 0; s < g.nsteps; ++s) {
-            const GStepDev st = gsteps[g.first_gstep + s];
-            const Dyn dd = dyn[(u64)li * nsteps + st.step];
-            const int ptile = (__popcll(tb & st.beta) & 1) ^ dd.ov;
-            coset_step(tile, st, dd, d, wb, ptile);
+        for (int pi = 0; pi < g.npasses; ++pi) {
+            const PassDev &ps = passes[g.first_pass + pi];
+            switch (ps.nmask) {
+                case 0: run_pass<S, 0, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
+                case 1: run_pass<S, 1, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
+                case 2: run_pass<S, 2, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
+                case 3: run_pass<S, 3, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
+                default: run_pass<S, 4, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
+            }
             __syncthreads();
         }
         for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
@@ -608,7 +531,7 @@ struct PendingEvent {
 struct dcs_plan {
     dcs_context *ctx;
     int storage, maxslots, lanes, nsteps, k, blocks;
-    int fuse_max_dim, resident_bits, tile_log2;
+    int fuse_max_dim, resident_bits, tile_log2, exact_rounding;
     size_t esz;
     u64 lane_stride;
     u64 terminal_mask;
@@ -625,7 +548,8 @@ struct dcs_plan {
     u64 *d_dmax = nullptr;
     StepDev *d_steps = nullptr;
     GroupDev *d_groups = nullptr;
-    GStepDev *d_gsteps = nullptr;
+    PassDev *d_passes = nullptr;
+    PStepDev *d_psteps = nullptr;
     u64 *d_addrv = nullptr;
     u64 *d_smap = nullptr;
     double *d_phase = nullptr;
@@ -729,8 +653,8 @@ bool step_is_resident(const StepHost &s, int rb) {
     return !(s.in_mask & ~lim) && !(s.out_mask & ~lim);
 }
 
-int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<GStepDev> &gsteps,
-                    std::vector<u64> &addrv) {
+int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<PassDev> &passes,
+                    std::vector<PStepDev> &psteps, std::vector<u64> &addrv) {
     const int n = p->nsteps;
     p->sched_steps.clear();
     for (int s = 0; s < n; ++s) p->sched_steps.push_back(Seg{0, s, 1, 0, -1, s});
@@ -788,13 +712,11 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<GSte
         g.d = d;
         g.wbits = wb;
         g.lane_identity = (low == ((wb == 64) ? ~0ull : ((1ull << wb) - 1)));
-        g.nsteps = e - s;
-        g.first_gstep = (int)gsteps.size();
         g.addrv_offset = (int)addrv.size();
         g.rep_low_mask = low;
         g.ntiles = 1ull << (rbits - wb);
         g.high = make_runs(R & ~low);
-        auto coords = [&](u64 m, uint32_t &out) -> bool {
+        auto tcoords = [&](u64 m, uint32_t &out) -> bool {
             uint32_t c = 0;
             u64 chk = 0;
             for (int i = 0; i < d; ++i)
@@ -825,41 +747,157 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<GSte
                 if ((t >> i) & 1) a ^= V[i];
             addrv.push_back(a);
         }
+        // Split the group into register passes: consecutive steps whose t-space masks stay linearly
+        // independent, at most three masks per pass.
+        struct TStep {
+            int j;
+            uint32_t m[2];
+            int nm;
+        };
+        std::vector<TStep> ts;
         for (int j = s; j < e; ++j) {
             const StepHost &st = p->steps[j];
-            GStepDev gs;
-            memset(&gs, 0, sizeof(gs));
-            gs.beta = st.beta;
-            gs.kind = st.kind;
-            gs.step = j;
-            gs.tp = gs.tpa = 0;
-            bool ok = true;
-            if (st.kind == 0) gs.tp = tindex(st.pivot);
-            if (st.kind == 3 || st.kind == 5 || st.kind == 6 || st.kind == 7) {
-                ok = ok && coords(st.K, gs.tk);
-                gs.tp = tindex(st.pivot);
-            }
-            if (st.kind == 1 || st.kind == 6) {
-                ok = ok && coords(st.Ka, gs.tka);
-                gs.tpa = tindex(st.pivot_a);
-                if (st.kind == 1) gs.tp = gs.tpa;
-            }
-            if (!ok || gs.tp < 0 || gs.tpa < 0)
-                return fail(DCS_ERR_INVALID, "step %d: orbit mask outside its group subspace", j);
-            for (int i = 0; i < d; ++i)
-                if (popc(V[i] & st.beta) & 1) gs.bvec |= 1u << i;
-            {
-                u64 r = low;
-                for (int jb = 0; jb < wb; ++jb) {
-                    const u64 bit = r & (~r + 1);
-                    if (st.beta & bit) gs.beta_lane |= 1u << jb;
-                    r &= r - 1;
+            TStep t{j, {0u, 0u}, 0};
+            uint32_t c = 0;
+            if (st.kind == 0) {
+                const int i = tindex(st.pivot);
+                if (i < 0) return fail(DCS_ERR_INVALID, "step %d: grow slot outside its group subspace", j);
+                t.m[t.nm++] = 1u << i;
+            } else if (st.kind == 1) {
+                if (!tcoords(st.Ka, c)) return fail(DCS_ERR_INVALID, "step %d: Ka outside its group subspace", j);
+                t.m[t.nm++] = c;
+            } else if (st.kind == 3 || st.kind == 5 || st.kind == 6 || st.kind == 7) {
+                if (!tcoords(st.K, c)) return fail(DCS_ERR_INVALID, "step %d: K outside its group subspace", j);
+                t.m[t.nm++] = c;
+                if (st.kind == 6) {
+                    if (!tcoords(st.Ka, c)) return fail(DCS_ERR_INVALID, "step %d: Ka outside its group subspace", j);
+                    t.m[t.nm++] = c;
                 }
             }
-            for (int i = 0; i < d; ++i)
-                if (!((st.in_mask >> top_bit(V[i])) & 1)) gs.inactive_t |= 1u << i;
-            gsteps.push_back(gs);
+            ts.push_back(t);
         }
+        auto top32 = [](uint32_t x) { return 31 - __builtin_clz(x); };
+        // Pass state: basis vectors (insertion order) and an echelon copy keyed by top bit that
+        // remembers which basis vectors each reduced vector combines.
+        struct PassState {
+            std::vector<uint32_t> basis;
+            std::vector<std::pair<uint32_t, uint32_t>> red;  // (reduced vector, combination of basis)
+        };
+        auto coords = [&](const PassState &P, uint32_t v, uint32_t &combo) -> bool {
+            combo = 0;
+            bool progress = true;
+            while (v && progress) {
+                progress = false;
+                for (const auto &rc : P.red)
+                    if ((v >> top32(rc.first)) & 1) {
+                        v ^= rc.first;
+                        combo ^= rc.second;
+                        progress = true;
+                    }
+            }
+            return v == 0;
+        };
+        auto add_basis = [&](PassState &P, uint32_t v) -> int {
+            uint32_t combo = 0;
+            uint32_t r = v;
+            bool progress = true;
+            while (r && progress) {
+                progress = false;
+                for (const auto &rc : P.red)
+                    if ((r >> top32(rc.first)) & 1) {
+                        r ^= rc.first;
+                        combo ^= rc.second;
+                        progress = true;
+                    }
+            }
+            const int idx = (int)P.basis.size();
+            P.basis.push_back(v);
+            P.red.push_back({r, combo | (1u << idx)});
+            return idx;
+        };
+        // Place one step into a pass state; returns false when it does not fit (span > kPassMax, or a
+        // quad whose K / Ka are not single basis vectors).
+        auto place = [&](PassState &P, const TStep &t, uint32_t &pat, uint32_t &pat2) -> bool {
+            pat = pat2 = 0;
+            for (int q = 0; q < t.nm; ++q) {
+                uint32_t c = 0;
+                if (!coords(P, t.m[q], c)) {
+                    if ((int)P.basis.size() == kPassMax) return false;
+                    c = 1u << add_basis(P, t.m[q]);
+                }
+                if (t.nm == 2 && __builtin_popcount(c) != 1) return false;
+                (q == 0 ? pat : pat2) = c;
+            }
+            return true;
+        };
+        g.first_pass = (int)passes.size();
+        size_t k0 = 0;
+        while (k0 < ts.size()) {
+            PassState P;
+            std::vector<std::pair<uint32_t, uint32_t>> pats;
+            size_t k1 = k0;
+            while (k1 < ts.size()) {
+                PassState trial = P;
+                uint32_t pat = 0, pat2 = 0;
+                if (!place(trial, ts[k1], pat, pat2)) {
+                    if (k1 == k0) return fail(DCS_ERR_INVALID, "step %d: orbit masks do not fit a pass", ts[k1].j);
+                    break;
+                }
+                P = trial;
+                pats.push_back({pat, pat2});
+                ++k1;
+            }
+            PassDev ps;
+            memset(&ps, 0, sizeof(ps));
+            ps.nmask = (int)P.basis.size();
+            ps.nsteps = (int)(k1 - k0);
+            ps.first_pstep = (int)psteps.size();
+            std::vector<int> piv;
+            for (const auto &rc : P.red) piv.push_back(top32(rc.first));
+            std::sort(piv.begin(), piv.end());
+            for (size_t i = 0; i < P.basis.size(); ++i) {
+                ps.piv[i] = piv[i];
+                ps.mw[i] = P.basis[i];
+            }
+            auto rolemask = [&](int bit) {
+                uint32_t m = 0;
+                for (size_t l = 0; l < P.basis.size(); ++l)
+                    if ((P.basis[l] >> bit) & 1) m |= 1u << l;
+                return m;
+            };
+            for (size_t k = k0; k < k1; ++k) {
+                const StepHost &st = p->steps[ts[k].j];
+                PStepDev q;
+                memset(&q, 0, sizeof(q));
+                q.beta = st.beta;
+                q.kind = st.kind;
+                q.step = ts[k].j;
+                q.pat = pats[k - k0].first;
+                q.pat2 = pats[k - k0].second;
+                for (int i = 0; i < d; ++i)
+                    if (popc(V[i] & st.beta) & 1) q.bvec |= 1u << i;
+                for (size_t l = 0; l < P.basis.size(); ++l)
+                    if (__builtin_popcount(P.basis[l] & q.bvec) & 1) q.lbv |= 1u << l;
+                if (ts[k].nm >= 1) q.pm = rolemask(top32(ts[k].m[0]));
+                if (ts[k].nm == 2) {
+                    q.pma = rolemask(top32(ts[k].m[1]));
+                    if ((ts[k].m[1] >> top32(ts[k].m[0])) & 1)
+                        return fail(DCS_ERR_INVALID, "step %d: Ka holds the K pivot", ts[k].j);
+                }
+                {
+                    u64 r = low;
+                    for (int jb = 0; jb < wb; ++jb) {
+                        const u64 bit = r & (~r + 1);
+                        if (st.beta & bit) q.beta_lane |= 1u << jb;
+                        r &= r - 1;
+                    }
+                }
+                psteps.push_back(q);
+            }
+            passes.push_back(ps);
+            k0 = k1;
+        }
+        g.npasses = (int)passes.size() - g.first_pass;
         p->sched_fused.push_back(Seg{2, s, e - s, d, (int)groups.size(), unit});
         groups.push_back(g);
         ++unit;
@@ -909,15 +947,16 @@ template <class S>
 int launch_coset(dcs_plan *p, int nl, const Seg &sg, int nunits, u64 ntiles) {
     S *E = (S *)p->frontier;
     dim3 grid((unsigned)std::max<u64>(1, std::min<u64>(ntiles, (u64)p->ctx->cus * 8)), nl);
-#define C(T)                                                                                         \
-    case T:                                                                                          \
-        PROF(2, (k_coset<S, T><<<grid, kThreads, 0, p->ctx->stream>>>(E, p->lane_stride, p->d_lanes, p->d_groups, \
-                                                                      sg.group, p->d_gsteps, p->d_addrv, p->d_dyn, \
-                                                                      p->nsteps, p->d_dmax, nunits, sg.unit))); \
-        break;
+#define C(T, X)                                                                                        \
+    PROF(2, (k_coset<S, T, X><<<grid, kThreads, 0, p->ctx->stream>>>(E, p->lane_stride, p->d_lanes, p->d_groups, \
+                                                                     sg.group, p->d_passes, p->d_psteps,        \
+                                                                     p->d_addrv, p->d_dyn, p->nsteps, p->d_dmax, \
+                                                                     nunits, sg.unit)))
+    const bool x = p->exact_rounding != 0;
     switch (p->tile_log2) {
-        C(10) C(11) C(12)
-        default: return fail(DCS_ERR_INVALID, "tile_log2 must be 10..12");
+        case 11: if (x) C(11, true); else C(11, false); break;
+        case 12: if (x) C(12, true); else C(12, false); break;
+        default: return fail(DCS_ERR_INVALID, "tile_log2 must be 11 or 12");
     }
 #undef C
     return DCS_OK;
@@ -1031,7 +1070,7 @@ void dcs_plan_destroy(dcs_plan *p) {
         (void)hipEventDestroy(pe.a);
         (void)hipEventDestroy(pe.b);
     }
-    void *dev[] = {p->frontier, p->d_dyn, p->d_lanes, p->d_dmax, p->d_steps, p->d_groups, p->d_gsteps,
+    void *dev[] = {p->frontier, p->d_dyn, p->d_lanes, p->d_dmax, p->d_steps, p->d_groups, p->d_passes, p->d_psteps,
                    p->d_addrv, p->d_smap, p->d_phase, p->d_blocks, p->d_cum, p->d_scratch, p->d_draw};
     for (void *q : dev)
         if (q) (void)hipFree(q);
@@ -1050,7 +1089,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     if (d->lanes < 1 || d->nsteps < 1 || d->blocks < 1) return fail(DCS_ERR_INVALID, "lanes, nsteps, blocks must be >= 1");
     if (d->k < 0 || d->k > 30 || popc(d->terminal_mask) > d->k)
         return fail(DCS_ERR_INVALID, "terminal mask holds more than k slots (k=%d)", d->k);
-    if (d->tile_log2 < 10 || d->tile_log2 > 12) return fail(DCS_ERR_INVALID, "tile_log2 must be 10..12");
+    if (d->tile_log2 < 11 || d->tile_log2 > 12) return fail(DCS_ERR_INVALID, "tile_log2 must be 11 or 12");
     if (d->resident_bits != 0 && (d->resident_bits < 6 || d->resident_bits > 12))
         return fail(DCS_ERR_INVALID, "resident_bits must be 0 or 6..12");
     if (d->fuse_max_dim < 0 || d->fuse_max_dim > kMaxDim) return fail(DCS_ERR_INVALID, "fuse_max_dim must be 0..8");
@@ -1069,6 +1108,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     p->fuse_max_dim = d->fuse_max_dim;
     p->resident_bits = d->resident_bits;
     p->tile_log2 = d->tile_log2;
+    p->exact_rounding = d->exact_rounding;
     p->esz = d->storage == DCS_STORAGE_C64 ? sizeof(float2) : sizeof(double2);
     p->lane_stride = 1ull << d->maxslots;
     p->terminal_mask = d->terminal_mask;
@@ -1132,10 +1172,11 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
         return bail(fail(DCS_ERR_INVALID, "terminal slots differ from the live slots after the last step"));
 
     std::vector<GroupDev> groups;
-    std::vector<GStepDev> gsteps;
+    std::vector<PassDev> passes;
+    std::vector<PStepDev> psteps;
     std::vector<u64> addrv;
     {
-        const int rc = build_schedules(p, groups, gsteps, addrv);
+        const int rc = build_schedules(p, groups, passes, psteps, addrv);
         if (rc) return bail(rc);
     }
 
@@ -1165,7 +1206,8 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     DALLOC(p->d_dmax, sizeof(u64) * (size_t)d->lanes * max_units);
     DALLOC(p->d_steps, sizeof(StepDev) * sdev.size());
     DALLOC(p->d_groups, sizeof(GroupDev) * std::max<size_t>(1, groups.size()));
-    DALLOC(p->d_gsteps, sizeof(GStepDev) * std::max<size_t>(1, gsteps.size()));
+    DALLOC(p->d_passes, sizeof(PassDev) * std::max<size_t>(1, passes.size()));
+    DALLOC(p->d_psteps, sizeof(PStepDev) * std::max<size_t>(1, psteps.size()));
     DALLOC(p->d_addrv, sizeof(u64) * std::max<size_t>(1, addrv.size()));
     DALLOC(p->d_smap, sizeof(u64) * block_len);
     DALLOC(p->d_phase, sizeof(double) * block_len);
@@ -1187,7 +1229,8 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     } while (0)
     H2D(p->d_steps, sdev.data(), sizeof(StepDev) * sdev.size());
     if (!groups.empty()) H2D(p->d_groups, groups.data(), sizeof(GroupDev) * groups.size());
-    if (!gsteps.empty()) H2D(p->d_gsteps, gsteps.data(), sizeof(GStepDev) * gsteps.size());
+    if (!passes.empty()) H2D(p->d_passes, passes.data(), sizeof(PassDev) * passes.size());
+    if (!psteps.empty()) H2D(p->d_psteps, psteps.data(), sizeof(PStepDev) * psteps.size());
     if (!addrv.empty()) H2D(p->d_addrv, addrv.data(), sizeof(u64) * addrv.size());
     H2D(p->d_smap, d->syndrome_map, sizeof(u64) * ((u64)1 << popc(d->terminal_mask)));
     H2D(p->d_phase, d->phase, sizeof(double) * block_len);

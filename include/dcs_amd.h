@@ -89,7 +89,10 @@ typedef struct {
     /* schedule knobs (DCS_MODE_FUSED) */
     int32_t fuse_max_dim;       /* largest coset dimension of a fused group, 1..8 */
     int32_t resident_bits;      /* steps whose live slots all sit below this bit run LDS-resident, <= 12 */
-    int32_t tile_log2;          /* entries per coset tile, 10..13 */
+    int32_t tile_log2;          /* log2 entries per coset tile, 11 or 12 */
+    int32_t exact_rounding;     /* 1: fused steps round to storage precision after every step (stored
+                                   mantissas equal the per-step schedule); 0: round once per register
+                                   pass of up to three steps (more accurate, fewer conversions) */
 } dcs_plan_desc;
 
 typedef struct {

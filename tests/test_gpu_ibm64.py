@@ -5,8 +5,10 @@
   reruns agree to 2.2e-16; the difference here is the FMA placement of the CUDA build), and the block
   mass with the audit record to 1e-10 relative.
 * complex64, plan inplace_k2_fp32 (4 slices, rmax 34): the draw index equals the recorded r1 draw
-  unless the recorded CDF margin is below 1e-7 (a reported tie); amplitudes at the drawn index agree
-  with the c128 scores to 1e-4 relative and block masses to 1e-5."""
+  unless the recorded CDF margin is below 1e-7 (a reported tie); block masses agree with the r1 masses
+  to 1e-5; amplitudes at the drawn index agree with the c128 scores to 1e-5 relative up to one global
+  phase per plan (the k2 and k3 parity operations differ by a pivot, whose local-complementation
+  identity holds up to a phase; the reference's audits compare |overlap| for the same reason)."""
 import numpy as np
 import pytest
 
@@ -52,6 +54,7 @@ def test_ibm64_c64_k2_draws_match_r1(ibm64, records):
     print('case ibm64 c64 k2 fused', flush=True)
     e = make(ibm64, 'inplace_k2_fp32', np.complex64, schedule='fused')
     ties = 0
+    phase = None
     for sid in range(3):
         s = records['samples'][str(sid)]
         vec, meta = e.block(bits(s['bits']))
@@ -64,6 +67,10 @@ def test_ibm64_c64_k2_draws_match_r1(ibm64, records):
             ties += 1
         else:
             assert d['A_index'] == s['A_index']
-        assert abs(v[A] - complex(re, im)) <= 1e-4 * abs(complex(re, im))
+        want = complex(re, im)
+        ratio = v[A] / want
+        assert abs(abs(ratio) - 1) <= 1e-5
+        phase = ratio if phase is None else phase
+        assert abs(ratio - phase) <= 1e-5
         assert abs(d['scaled_mass_cdf'] - s['mass']) <= 1e-5
     e.close()
