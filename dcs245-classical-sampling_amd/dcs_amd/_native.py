@@ -39,7 +39,7 @@ class PlanDesc(C.Structure):
                 ('syndrome_map', C.POINTER(C.c_uint64)), ('phase', C.POINTER(C.c_double)),
                 ('open_unitaries', C.POINTER(C.c_double)), ('blocks', C.c_int32),
                 ('fuse_max_dim', C.c_int32), ('resident_bits', C.c_int32), ('tile_log2', C.c_int32),
-                ('exact_rounding', C.c_int32)]
+                ('exact_rounding', C.c_int32), ('pass_max_dim', C.c_int32)]
 
 
 class DrawResult(C.Structure):
@@ -157,7 +157,8 @@ class Plan:
     """Device-resident plan: frontier buffer(s), step tables, syndrome map, phase, block accumulators."""
 
     def __init__(self, ctx, table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, phase,
-                 open_unitaries, blocks=1, fuse_max_dim=6, resident_bits=12, tile_log2=12, exact_rounding=False):
+                 open_unitaries, blocks=1, fuse_max_dim=6, resident_bits=12, tile_log2=11, exact_rounding=False,
+                 pass_max_dim=3):
         self.ctx, self.lib = ctx, ctx.lib
         n = len(table['kind'])
         keep = {}
@@ -185,6 +186,7 @@ class Plan:
         d.open_unitaries = ptr(U, C.c_double)
         d.blocks, d.fuse_max_dim, d.resident_bits, d.tile_log2 = int(blocks), int(fuse_max_dim), int(resident_bits), int(tile_log2)
         d.exact_rounding = int(bool(exact_rounding))
+        d.pass_max_dim = int(pass_max_dim)
         h = C.c_void_p()
         check(self.lib.dcs_plan_create(ctx.handle, C.byref(d), C.byref(h)))
         self.handle = h

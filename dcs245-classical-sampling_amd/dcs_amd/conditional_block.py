@@ -90,7 +90,8 @@ class InplaceConditionalBlockGPU:
         self._plan = None
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='fused',
-              fuse_max_dim=6, resident_bits=12, tile_log2=12, blocks=1, exact_rounding=False):
+              fuse_max_dim=6, resident_bits=12, tile_log2=11, blocks=1, exact_rounding=False,
+              pass_max_dim=3):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
@@ -118,7 +119,8 @@ class InplaceConditionalBlockGPU:
         self._plan = _native.Plan(self.ctx, bp.step_table(), bp.inplace_maxslots, storage, self.lanes, bp.k,
                                   bp.terminal_mask(), bp.syndrome_map(), bp.phase(), bp.open_unitaries(),
                                   blocks=blocks, fuse_max_dim=fuse_max_dim, resident_bits=resident_bits,
-                                  tile_log2=tile_log2, exact_rounding=exact_rounding)
+                                  tile_log2=tile_log2, exact_rounding=exact_rounding,
+                                  pass_max_dim=pass_max_dim)
         self.buf = True
 
     # ---- reference mixin hooks -------------------------------------------------------------

@@ -224,9 +224,11 @@ __device__ __forceinline__ void make_index(uint32_t (&idx)[1 << J], const uint32
     }
 }
 
-// One pass over the tile.  tb: physical tile base; dyn_lane: this lane's per-step parameters.
+// One pass over the tile (held in FP64 in LDS).  S is the frontier storage type: with EXACT every
+// step's results are rounded to it, as the per-step kernel does; otherwise values stay FP64 until the
+// group is written back.  tb: physical tile base; dyn_lane: this lane's per-step parameters.
 template <class S, int J, bool EXACT>
-__device__ __forceinline__ void run_pass(S *tile, const PassDev &ps, const PStepDev *__restrict__ psteps,
+__device__ __forceinline__ void run_pass(double2 *tile, const PassDev &ps, const PStepDev *__restrict__ psteps,
                                          const void *dyn_lane, int dyn_stride, int d, int wb, u64 tb) {
     constexpr int NE = 1 << J;
     const uint32_t W = 1u << wb;
@@ -244,7 +246,7 @@ __device__ __forceinline__ void run_pass(S *tile, const PassDev &ps, const PStep
         make_index<J>(idx, mw);
         cd x[NE];
 #pragma unroll
-        for (int e = 0; e < NE; ++e) x[e] = Store<S>::load(tile[idx[e]]);
+        for (int e = 0; e < NE; ++e) x[e] = Store<double2>::load(tile[idx[e]]);
         for (int s = 0; s < ps.nsteps; ++s) {
             const PStepDev &st = psteps[ps.first_pstep + s];
             const double *dd = (const double *)((const char *)dyn_lane + (size_t)st.step * dyn_stride);
@@ -267,6 +269,6 @@ __device__ __forceinline__ void run_pass(S *tile, const PassDev &ps, const PStep
             }
         }
 #pragma unroll
-        for (int e = 0; e < NE; ++e) tile[idx[e]] = Store<S>::round(x[e]);
+        for (int e = 0; e < NE; ++e) tile[idx[e]] = Store<double2>::round(x[e]);
     }
 }

@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(kThreads) k_resident(S *__restrict__ base, u64
 // ------------------------------------------------------------------------------------------------
 // k_coset: fused group.  Tile layout: element (t, l) at LDS index t*W + l, t in [0, 2^d) the
 // coordinate in the RREF basis of V (address = c0(l) ^ addrv[t]), l the lane coordinate.
-template <class S, int TL, bool EXACT>
+template <class S, int TL, bool EXACT, int JMAX>
 __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 lane_stride,
                                                     const int *__restrict__ lane_ids,
                                                     const GroupDev *__restrict__ groups, int gi,
@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 la
                                                     const u64 *__restrict__ addrv_all,
                                                     const Dyn *__restrict__ dyn, int nsteps,
                                                     u64 *__restrict__ dmax, int nunits, int unit) {
-    __shared__ S tile[1 << TL];
+    __shared__ double2 tile[1 << TL];
     __shared__ u64 addrv[1 << kMaxDim];
     const GroupDev &g = groups[gi];
     const int li = blockIdx.y;
@@ -346,26 +346,55 @@ __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 la
     const double scale = scale_from_m2(prev);
     double lm = 0.0;
     __syncthreads();
-    for (u64 ti = blockIdx.x; ti < g.ntiles; ti += gridDim.x) {
-        const u64 tb = deposit(ti, g.high);
-        for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
-            const uint32_t t = e >> wb, l = e & (W - 1);
+    // Software pipeline: the next tile's HBM loads are issued into registers before the current
+    // tile's passes run, so each workgroup overlaps its own memory traffic with its LDS/VALU work.
+    constexpr int PER = (1 << TL) / kThreads;
+    S R[PER];
+    auto issue = [&](u64 tbase) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t e = threadIdx.x + k * kThreads;
             S v = Store<S>::zero();
-            if (!(t & g.dead_in_t)) {
-                const u64 lp = g.lane_identity ? (u64)l : pdep_loop(l, g.rep_low_mask);
-                v = E[tb ^ lp ^ addrv[t]];
+            if (e < n) {
+                const uint32_t t = e >> wb, l = e & (W - 1);
+                if (!(t & g.dead_in_t)) {
+                    const u64 lp = g.lane_identity ? (u64)l : pdep_loop(l, g.rep_low_mask);
+                    v = E[tbase ^ lp ^ addrv[t]];
+                }
             }
-            tile[e] = v;
+            R[k] = v;
+        }
+    };
+    u64 ti = blockIdx.x;
+    u64 tb = 0;
+    if (ti < g.ntiles) {
+        tb = deposit(ti, g.high);
+        issue(tb);
+    }
+    while (ti < g.ntiles) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t e = threadIdx.x + k * kThreads;
+            if (e < n) {
+                const cd v = Store<S>::load(R[k]);
+                tile[e] = make_double2(v.r, v.i);
+            }
         }
         __syncthreads();
+        const u64 tn = ti + gridDim.x;
+        u64 tb_next = 0;
+        if (tn < g.ntiles) {
+            tb_next = deposit(tn, g.high);
+            issue(tb_next);
+        }
         for (int pi = 0; pi < g.npasses; ++pi) {
             const PassDev &ps = passes[g.first_pass + pi];
             switch (ps.nmask) {
                 case 0: run_pass<S, 0, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
                 case 1: run_pass<S, 1, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
                 case 2: run_pass<S, 2, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
-                case 3: run_pass<S, 3, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
-                default: run_pass<S, 4, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
+                case 3: if constexpr (JMAX >= 3) run_pass<S, 3, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
+                default: if constexpr (JMAX >= 4) run_pass<S, 4, EXACT>(tile, ps, psteps, dyn_lane, sizeof(Dyn), d, wb, tb); break;
             }
             __syncthreads();
         }
@@ -373,9 +402,11 @@ __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 la
             const uint32_t t = e >> wb, l = e & (W - 1);
             if (t & g.dead_out_t) continue;
             const u64 lp = g.lane_identity ? (u64)l : pdep_loop(l, g.rep_low_mask);
-            put_scaled(E, tb ^ lp ^ addrv[t], Store<S>::load(tile[e]), scale, lm);
+            put_scaled(E, tb ^ lp ^ addrv[t], Store<double2>::load(tile[e]), scale, lm);
         }
         __syncthreads();
+        ti = tn;
+        tb = tb_next;
     }
     commit_max(lm, &dmax[(u64)li * nunits + unit]);
 }
@@ -531,7 +562,7 @@ struct PendingEvent {
 struct dcs_plan {
     dcs_context *ctx;
     int storage, maxslots, lanes, nsteps, k, blocks;
-    int fuse_max_dim, resident_bits, tile_log2, exact_rounding;
+    int fuse_max_dim, resident_bits, tile_log2, exact_rounding, pass_max_dim;
     size_t esz;
     u64 lane_stride;
     u64 terminal_mask;
@@ -822,7 +853,7 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
             for (int q = 0; q < t.nm; ++q) {
                 uint32_t c = 0;
                 if (!coords(P, t.m[q], c)) {
-                    if ((int)P.basis.size() == kPassMax) return false;
+                    if ((int)P.basis.size() >= p->pass_max_dim) return false;
                     c = 1u << add_basis(P, t.m[q]);
                 }
                 if (t.nm == 2 && __builtin_popcount(c) != 1) return false;
@@ -947,16 +978,23 @@ template <class S>
 int launch_coset(dcs_plan *p, int nl, const Seg &sg, int nunits, u64 ntiles) {
     S *E = (S *)p->frontier;
     dim3 grid((unsigned)std::max<u64>(1, std::min<u64>(ntiles, (u64)p->ctx->cus * 8)), nl);
-#define C(T, X)                                                                                        \
-    PROF(2, (k_coset<S, T, X><<<grid, kThreads, 0, p->ctx->stream>>>(E, p->lane_stride, p->d_lanes, p->d_groups, \
-                                                                     sg.group, p->d_passes, p->d_psteps,        \
-                                                                     p->d_addrv, p->d_dyn, p->nsteps, p->d_dmax, \
-                                                                     nunits, sg.unit)))
+#define C(T, X, JM)                                                                                     \
+    PROF(2, (k_coset<S, T, X, JM><<<grid, kThreads, 0, p->ctx->stream>>>(E, p->lane_stride, p->d_lanes,      \
+                                                                         p->d_groups, sg.group, p->d_passes, \
+                                                                         p->d_psteps, p->d_addrv, p->d_dyn,  \
+                                                                         p->nsteps, p->d_dmax, nunits, sg.unit)))
     const bool x = p->exact_rounding != 0;
-    switch (p->tile_log2) {
-        case 11: if (x) C(11, true); else C(11, false); break;
-        case 12: if (x) C(12, true); else C(12, false); break;
-        default: return fail(DCS_ERR_INVALID, "tile_log2 must be 11 or 12");
+    const int key = p->tile_log2 * 100 + (x ? 10 : 0) + p->pass_max_dim;
+    switch (key) {
+        case 1203: C(12, false, 3); break;
+        case 1204: C(12, false, 4); break;
+        case 1213: C(12, true, 3); break;
+        case 1214: C(12, true, 4); break;
+        case 1103: C(11, false, 3); break;
+        case 1104: C(11, false, 4); break;
+        case 1113: C(11, true, 3); break;
+        case 1114: C(11, true, 4); break;
+        default: return fail(DCS_ERR_INVALID, "unsupported tile_log2 %d / pass_max_dim %d", p->tile_log2, p->pass_max_dim);
     }
 #undef C
     return DCS_OK;
@@ -1093,6 +1131,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     if (d->resident_bits != 0 && (d->resident_bits < 6 || d->resident_bits > 12))
         return fail(DCS_ERR_INVALID, "resident_bits must be 0 or 6..12");
     if (d->fuse_max_dim < 0 || d->fuse_max_dim > kMaxDim) return fail(DCS_ERR_INVALID, "fuse_max_dim must be 0..8");
+    if (d->pass_max_dim < 3 || d->pass_max_dim > kPassMax) return fail(DCS_ERR_INVALID, "pass_max_dim must be 3 or 4");
     const u64 lim = (d->maxslots == 0) ? 1ull : ((1ull << d->maxslots) - 1);
     if (d->terminal_mask & ~lim) return fail(DCS_ERR_INVALID, "terminal slots beyond maxslots");
     HIPCHK(hipSetDevice(ctx->device));
@@ -1109,6 +1148,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     p->resident_bits = d->resident_bits;
     p->tile_log2 = d->tile_log2;
     p->exact_rounding = d->exact_rounding;
+    p->pass_max_dim = d->pass_max_dim;
     p->esz = d->storage == DCS_STORAGE_C64 ? sizeof(float2) : sizeof(double2);
     p->lane_stride = 1ull << d->maxslots;
     p->terminal_mask = d->terminal_mask;

@@ -92,7 +92,8 @@ typedef struct {
     int32_t tile_log2;          /* log2 entries per coset tile, 11 or 12 */
     int32_t exact_rounding;     /* 1: fused steps round to storage precision after every step (stored
                                    mantissas equal the per-step schedule); 0: round once per register
-                                   pass of up to three steps (more accurate, fewer conversions) */
+                                   pass (more accurate, fewer conversions) */
+    int32_t pass_max_dim;       /* span of the orbit masks of one register pass, 3 or 4 */
 } dcs_plan_desc;
 
 typedef struct {

@@ -62,19 +62,20 @@ def test_c1_c64_steps_bit_exact(c1):
     e.close()
 
 
-@pytest.mark.parametrize('rb,dim,tl', [(12, 6, 12), (6, 6, 12), (6, 8, 11), (0, 4, 11), (6, 1, 12)])
-def test_c1_fused_matches_steps(c1, rb, dim, tl):
+@pytest.mark.parametrize('rb,dim,tl,pj', [(12, 6, 12, 3), (6, 6, 12, 4), (6, 8, 11, 3), (0, 4, 11, 4), (6, 1, 12, 3)])
+def test_c1_fused_matches_steps(c1, rb, dim, tl, pj):
     """complex128 fused and complex64 fused with per-step rounding keep the per-step mantissas
     (differences only through exp(ls)); complex64 fused with per-pass rounding stays within 1e-5 of
     FP64 and no further from it than the per-step complex64 schedule (x2 margin)."""
     print(f'case c1 fused rb={rb} dim={dim} tl={tl}', flush=True)
     for dt, ref, tol, exact in ((np.complex128, c1['ref128'], 1e-13, False), (np.complex64, c1['ref64'], 1e-13, True)):
         e = engine(c1['args'], dtype=dt, schedule='fused', resident_bits=rb, fuse_max_dim=dim, tile_log2=tl,
-                   exact_rounding=exact)
+                   exact_rounding=exact, pass_max_dim=pj)
         v, _ = e.block(c1['x'])
         assert rel(v, ref) < tol, (dt, rel(v, ref))
         e.close()
-    e = engine(c1['args'], dtype=np.complex64, schedule='fused', resident_bits=rb, fuse_max_dim=dim, tile_log2=tl)
+    e = engine(c1['args'], dtype=np.complex64, schedule='fused', resident_bits=rb, fuse_max_dim=dim, tile_log2=tl,
+               pass_max_dim=pj)
     v, _ = e.block(c1['x'])
     err, err_steps = rel(v, c1['ref128']), rel(c1['ref64'], c1['ref128'])
     print(f'c64 fused error {err:.3e}, per-step c64 error {err_steps:.3e}', flush=True)
