@@ -95,7 +95,7 @@ def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False):
     return info
 
 
-def cpu_baseline(max_width=28):
+def cpu_baseline(max_width=32):
     """C oracle, complex64 storage, on the first steps of one k2 slice whose live width is <= max_width."""
     from oracle import oracle
     from dcs_amd.block_plan import BlockPlan
@@ -114,6 +114,7 @@ def cpu_baseline(max_width=28):
         m += 1
     nbytes = sum(((1 << len(s.active_in)) + (1 << len(s.active_out))) * 8 for s in bp.inplace_steps[:m])
     E = np.zeros(1 << max_width, np.complex64)
+    E[:] = 0  # touch every page before the timed region
     m2 = np.zeros(m, np.float64)
     arr = {k: np.ascontiguousarray(t[k][:m]) for k in ('kind', 'K', 'Ka', 'pivot', 'pivot_a', 'beta', 'in_mask')}
     P = lambda a, ct: a.ctypes.data_as(C.POINTER(ct))
@@ -151,6 +152,14 @@ def main():
     from dcs_amd import _native
     dev = _native.device_info(0)
     log('device', dev)
+    # streaming copy of 4 GiB (read + write): the practical HBM ceiling, and the calibration kernel
+    # (known bytes) for the FETCH_SIZE / WRITE_SIZE counter runs
+    ctx = _native.Context(0)
+    copy_bytes = 4 << 30
+    copy_ms = ctx.stream_copy_ms(copy_bytes, 5)
+    ctx.close()
+    stream_GBps = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
+    log(f'stream copy: {stream_GBps:.1f} GB/s')
 
     log('fast path: complex64, k2 plan, schedule', a.schedule)
     fast = run_path('inplace_k2_fp32', np.complex64, a.schedule, a.steps, a.warmup, profile=True)
@@ -187,6 +196,10 @@ def main():
         cpu = cpu_baseline()
         log('cpu', cpu)
 
+    traffic = None
+    tf = ROOT / 'profiles' / 'k_coset_ibm64_k2_c64_traffic.json'
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get('hbm_bytes_per_launch')
     lat = np.array(fast['lat']) * 1e3
     line = {
         'metric': 'samples/s @64q 2^20 block: batched c64 vs plain c128, HBM GB/s; p50/p99 ms',
@@ -201,10 +214,12 @@ def main():
         'effective_hbm_GBps': unfused_bytes * sps / 1e9,
         'effective_roofline_frac': unfused_bytes * sps / HBM_PEAK,
         'algorithmic_bytes_per_sample': {'unfused': unfused_bytes, 'fused': sum(fb.values())},
+        'stream_copy_GBps': stream_GBps,
         'plain_c128': plain,
         'speedup_vs_plain_c128': (sps / plain['samples_per_s']) if plain else None,
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9 if achieved else None, 'peak': HBM_PEAK / 1e9,
-                     'unit': 'GB/s', 'frac': (achieved / HBM_PEAK) if achieved else None, 'traffic': None,
+                     'unit': 'GB/s', 'frac': (achieved / HBM_PEAK) if achieved else None, 'traffic': traffic,
+                     'launches_per_sample': int(launches[2]),
                      'kernel': 'k_coset', 'bytes_per_sample': coset_bytes, 'ms_per_sample': coset_ms},
         'cpu_baseline': cpu,
         'device': {k: dev[k] for k in ('device_name', 'arch', 'compute_units', 'total_bytes')},

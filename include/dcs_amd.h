@@ -7,7 +7,7 @@
  * library; the host passes plain arrays.  Every entry point returns a dcs_status; the message of the
  * last failure on the calling thread is available from dcs_last_error().
  *
- * Reference interfaces replaced (paths relative to the reference checkout):
+ * Reference interfaces replaced (paths relative to the reference's source/ directory):
  *   dcs_context_create        cp.cuda.Device(device).use()            kernel_opt/inplace_gpu.py:221-222
  *   dcs_get_device_info       getDeviceProperties / runtime versions  production/worker.py:60-76
  *   dcs_plan_create           InplaceGPUMixin.alloc (plan upload,     kernel_opt/inplace_gpu.py:220-240

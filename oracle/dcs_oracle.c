@@ -74,6 +74,25 @@ static inline double scale_of(double m2) {
     return ldexp(1.0, -e);
 }
 
+/* contiguous runs of a slot mask: bits [src, src+len) of the orbit index go to [dst, dst+len) */
+typedef struct { int n; int src[64], len[64], dst[64]; } runs_t;
+static void make_runs(u64 mask, runs_t *r) {
+    int src = 0;
+    r->n = 0;
+    for (int p = 0; p < 64;) {
+        if (!((mask >> p) & 1)) { ++p; continue; }
+        int q = p;
+        while (q < 64 && ((mask >> q) & 1)) ++q;
+        r->src[r->n] = src; r->len[r->n] = q - p; r->dst[r->n] = p;
+        src += q - p; r->n++; p = q;
+    }
+}
+static inline u64 deposit(u64 x, const runs_t *r) {
+    u64 c = 0;
+    for (int i = 0; i < r->n; ++i) c |= ((x >> r->src[i]) & ((1ull << r->len[i]) - 1)) << r->dst[i];
+    return c;
+}
+
 static u64 orbit_slots(int kind, u64 in_mask, int pivot, int pivot_a) {
     u64 m = in_mask;
     if (kind == 1) m &= ~(1ull << pivot_a);
@@ -102,10 +121,12 @@ int oracle_contract(int storage, int nsteps, const int32_t *kind, const u64 *K, 
         const u64 b = beta[s], k = K[s], ka = Ka[s];
         const int pv = pivot[s];
         const double scale = scale_of(prev);
+        runs_t runs;
+        make_runs(slots, &runs);
         double m = 0.0;
 #pragma omp parallel for schedule(static) reduction(max : m)
         for (long long x = 0; x < norb; ++x) {
-            const u64 c = pdep((u64)x, slots);
+            const u64 c = deposit((u64)x, &runs);
             const cd a = ld(E, storage, c);
             const int sc = sbit(c, b, o);
             double lm = 0.0;
