@@ -14,11 +14,12 @@ LIB_PATH = Path(__file__).resolve().parent / 'libdcs_amd.so'
 
 DCS_OK, DCS_ERR_INVALID, DCS_ERR_RUNTIME, DCS_ERR_NOMEM, DCS_ERR_NONFINITE, DCS_ERR_ZERO = range(6)
 STORAGE_C64, STORAGE_C128 = 0, 1
-MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED = 0, 1, 2
+MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED, MODE_COMPILED = 0, 1, 2, 3
 
 EXPORTED = (
     'dcs_last_error', 'dcs_get_device_count', 'dcs_get_device_info', 'dcs_context_create',
-    'dcs_context_destroy', 'dcs_synchronize', 'dcs_plan_create', 'dcs_plan_destroy', 'dcs_plan_units',
+    'dcs_context_destroy', 'dcs_synchronize', 'dcs_plan_create', 'dcs_plan_destroy', 'dcs_plan_compile',
+    'dcs_plan_units',
     'dcs_plan_schedule', 'dcs_contract', 'dcs_block_zero', 'dcs_block_accumulate', 'dcs_block_transform',
     'dcs_block_read', 'dcs_block_device_pointer', 'dcs_draw', 'dcs_frontier_read', 'dcs_plan_profile',
     'dcs_stream_copy')
@@ -39,7 +40,7 @@ class PlanDesc(C.Structure):
                 ('syndrome_map', C.POINTER(C.c_uint64)), ('phase', C.POINTER(C.c_double)),
                 ('open_unitaries', C.POINTER(C.c_double)), ('blocks', C.c_int32),
                 ('fuse_max_dim', C.c_int32), ('resident_bits', C.c_int32), ('tile_log2', C.c_int32),
-                ('exact_rounding', C.c_int32), ('pass_max_dim', C.c_int32)]
+                ('exact_rounding', C.c_int32), ('pass_max_dim', C.c_int32), ('lds_storage', C.c_int32)]
 
 
 class DrawResult(C.Structure):
@@ -69,6 +70,7 @@ def load():
         'dcs_synchronize': (C.c_int, [vp]),
         'dcs_plan_create': (C.c_int, [vp, P(PlanDesc), P(vp)]),
         'dcs_plan_destroy': (None, [vp]),
+        'dcs_plan_compile': (C.c_int, [vp, P(dbl)]),
         'dcs_plan_units': (C.c_int, [vp, C.c_int, P(C.c_int)]),
         'dcs_plan_schedule': (C.c_int, [vp, C.c_int, C.c_int, P(i32), P(i32), P(i32), P(i32), P(C.c_int)]),
         'dcs_contract': (C.c_int, [vp, C.c_int, C.c_int, P(i32), P(dbl), P(i32), P(i32), P(dbl)]),
@@ -158,7 +160,7 @@ class Plan:
 
     def __init__(self, ctx, table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, phase,
                  open_unitaries, blocks=1, fuse_max_dim=6, resident_bits=12, tile_log2=11, exact_rounding=False,
-                 pass_max_dim=3):
+                 pass_max_dim=3, lds_storage=False):
         self.ctx, self.lib = ctx, ctx.lib
         n = len(table['kind'])
         keep = {}
@@ -187,12 +189,20 @@ class Plan:
         d.blocks, d.fuse_max_dim, d.resident_bits, d.tile_log2 = int(blocks), int(fuse_max_dim), int(resident_bits), int(tile_log2)
         d.exact_rounding = int(bool(exact_rounding))
         d.pass_max_dim = int(pass_max_dim)
+        d.lds_storage = int(bool(lds_storage))
         h = C.c_void_p()
         check(self.lib.dcs_plan_create(ctx.handle, C.byref(d), C.byref(h)))
         self.handle = h
         self.nsteps, self.k, self.lanes, self.blocks = n, int(k), int(lanes), int(blocks)
         self.storage = int(storage)
         self.terminal_rank = bin(int(terminal_mask)).count('1')
+
+    def compile(self):
+        """Generate and compile the plan-specialised group kernels; returns the compile seconds
+        (0.0 when the module was already cached in this process)."""
+        sec = C.c_double(0.0)
+        check(self.lib.dcs_plan_compile(self.handle, C.byref(sec)))
+        return sec.value
 
     def units(self, mode):
         u = C.c_int()

@@ -22,7 +22,8 @@ import numpy as np
 from . import _native
 from .block_plan import BlockPlan
 
-SCHEDULES = {'steps': _native.MODE_STEPS, 'steps_host': _native.MODE_STEPS_HOST, 'fused': _native.MODE_FUSED}
+SCHEDULES = {'steps': _native.MODE_STEPS, 'steps_host': _native.MODE_STEPS_HOST, 'fused': _native.MODE_FUSED,
+             'compiled': _native.MODE_COMPILED}
 
 
 def logscale_from_maxima(m2):
@@ -91,15 +92,20 @@ class InplaceConditionalBlockGPU:
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='fused',
               fuse_max_dim=6, resident_bits=12, tile_log2=11, blocks=1, exact_rounding=False,
-              pass_max_dim=3):
+              pass_max_dim=3, lds_storage=False):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
         lanes      frontiers held at once; slices of one block are contracted `lanes` at a time
-        schedule   'fused' (resident runs + coset groups), 'steps' (one launch per step, scale chain
-                   on the device) or 'steps_host' (one launch per step, host reads every maximum)
-        exact_rounding  fused schedule only: round to storage precision after every step (mantissas equal
+        schedule   'compiled' (resident runs + coset groups, one generated kernel per group, compiled
+                   here with hipRTC), 'fused' (the same schedule on the generic coset kernel), 'steps'
+                   (one launch per step, scale chain on the device) or 'steps_host' (one launch per
+                   step, host reads every maximum)
+        exact_rounding  fused schedules: round to storage precision after every step (mantissas equal
                    the per-step schedule) instead of once per register pass of up to three steps
+        lds_storage  compiled schedule: hold coset tiles in LDS in storage precision (half the LDS of
+                   FP64 tiles for complex64, so twice the workgroups per CU); values are then rounded
+                   to storage once per register pass
         The legacy fp64_kernel_path argument is accepted and ignored, as in the reference."""
         dt = np.dtype(dtype)
         if dt not in (np.dtype(np.complex64), np.dtype(np.complex128)):
@@ -120,7 +126,8 @@ class InplaceConditionalBlockGPU:
                                   bp.terminal_mask(), bp.syndrome_map(), bp.phase(), bp.open_unitaries(),
                                   blocks=blocks, fuse_max_dim=fuse_max_dim, resident_bits=resident_bits,
                                   tile_log2=tile_log2, exact_rounding=exact_rounding,
-                                  pass_max_dim=pass_max_dim)
+                                  pass_max_dim=pass_max_dim, lds_storage=lds_storage)
+        self.compile_seconds = self._plan.compile() if self.mode == _native.MODE_COMPILED else 0.0
         self.buf = True
 
     # ---- reference mixin hooks -------------------------------------------------------------

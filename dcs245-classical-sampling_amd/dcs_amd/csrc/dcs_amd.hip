@@ -20,12 +20,16 @@
 //               of draw_checked (sampling/sampling_protocol.py:56-65); the single-wave sequential
 //               scan reproduces numpy's accumulation order bit for bit.
 #include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -99,12 +103,6 @@ __device__ __forceinline__ u64 deposit(u64 x, const Runs &r) {
     return c;
 }
 
-struct Dyn {
-    double f0r, f0i, f1r, f1i;
-    int32_t ov, y;
-    int32_t pad[2];
-};
-
 struct StepArgs {
     u64 norbits, K, Ka, beta;
     int32_t kind, pivot;
@@ -131,22 +129,6 @@ struct DrawOut {
     int32_t status;
     int32_t pad;
 };
-
-__device__ __forceinline__ double bits2d(u64 b) { return __longlong_as_double((long long)b); }
-
-// Wave-level max, then one atomic per wave (values are >= 0 or NaN, so the u64 order of the
-// float64 bit patterns is their numeric order, NaN above +inf).
-__device__ __forceinline__ void commit_max(double lm, u64 *slot) {
-    for (int off = 32; off > 0; off >>= 1) lm = max_nan(lm, __shfl_xor(lm, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(slot, (u64)__double_as_longlong(lm));
-}
-
-template <class S>
-__device__ __forceinline__ void put_scaled(S *E, u64 c, cd v, double scale, double &lm) {
-    const S o = Store<S>::round(cd{v.r * scale, v.i * scale});
-    E[c] = o;
-    lm = max_nan(lm, Store<S>::norm2(o));
-}
 
 // ------------------------------------------------------------------------------------------------
 // k_step: one step, one thread per orbit, orbit index deposited into the enumerated slots.
@@ -548,6 +530,15 @@ inline int popc(u64 x) { return __builtin_popcountll(x); }
 
 }  // namespace
 
+#include "group_codegen.hpp"
+
+namespace {
+// Text of dcs_kernels.hpp, compiled into every generated module.
+const char kKernelsHpp[] = {
+#embed "dcs_kernels.hpp"
+    , 0};
+}  // namespace
+
 struct dcs_context {
     int device;
     hipStream_t stream;
@@ -562,7 +553,7 @@ struct PendingEvent {
 struct dcs_plan {
     dcs_context *ctx;
     int storage, maxslots, lanes, nsteps, k, blocks;
-    int fuse_max_dim, resident_bits, tile_log2, exact_rounding, pass_max_dim;
+    int fuse_max_dim, resident_bits, tile_log2, exact_rounding, pass_max_dim, lds_storage;
     size_t esz;
     u64 lane_stride;
     u64 terminal_mask;
@@ -571,6 +562,15 @@ struct dcs_plan {
     std::vector<double> unitaries;  // k*8
     std::vector<Seg> sched_steps, sched_fused;
     std::vector<u64> group_ntiles;
+    // host copies of the fused-schedule tables (source of the generated kernels)
+    std::vector<GroupDev> h_groups;
+    std::vector<PassDev> h_passes;
+    std::vector<PStepDev> h_psteps;
+    std::vector<u64> h_addrv;
+    // DCS_MODE_COMPILED
+    bool compiled = false;
+    std::vector<hipFunction_t> group_fn;
+    std::vector<unsigned> group_grid;
     int units_steps, units_fused;
     // device memory
     void *frontier = nullptr;
@@ -1000,10 +1000,95 @@ int launch_coset(dcs_plan *p, int nl, const Seg &sg, int nunits, u64 ntiles) {
     return DCS_OK;
 }
 
+// ---- DCS_MODE_COMPILED: generated group kernels -------------------------------------------------
+struct ModuleCache {
+    std::mutex mu;
+    std::map<std::pair<int, std::string>, hipModule_t> mods;  // (device, source) -> loaded module
+};
+ModuleCache &module_cache() {
+    static ModuleCache c;
+    return c;
+}
+
+int compile_groups(dcs_plan *p, double *seconds) {
+    if (seconds) *seconds = 0.0;
+    if (p->compiled) return DCS_OK;
+    CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0};
+    std::string src = codegen::prelude(opt, kKernelsHpp);
+    for (size_t gi = 0; gi < p->h_groups.size(); ++gi) {
+        const GroupDev &g = p->h_groups[gi];
+        codegen::GroupView v{&g, p->h_passes.data() + g.first_pass, p->h_psteps.data(), p->h_addrv.data() + g.addrv_offset};
+        src += codegen::group_kernel(opt, (int)gi, v);
+    }
+    HIPCHK(hipSetDevice(p->ctx->device));
+    hipModule_t mod = nullptr;
+    {
+        ModuleCache &mc = module_cache();
+        std::lock_guard<std::mutex> lk(mc.mu);
+        auto key = std::make_pair(p->ctx->device, src);
+        auto it = mc.mods.find(key);
+        if (it != mc.mods.end()) {
+            mod = it->second;
+        } else {
+            const auto t0 = std::chrono::steady_clock::now();
+            hiprtcProgram prog;
+            if (hiprtcCreateProgram(&prog, src.c_str(), "dcs_groups.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+                return fail(DCS_ERR_RUNTIME, "hiprtcCreateProgram failed");
+            const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                                  "-Wno-pragma-once-outside-header", "-Xclang", "-target-feature", "-Xclang",
+                                  "-packed-fp32-ops"};
+            const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+            if (rc != HIPRTC_SUCCESS) {
+                size_t ls = 0;
+                hiprtcGetProgramLogSize(prog, &ls);
+                std::string log(ls, '\0');
+                if (ls) hiprtcGetProgramLog(prog, &log[0]);
+                hiprtcDestroyProgram(&prog);
+                return fail(DCS_ERR_RUNTIME, "hipRTC compile of the group kernels failed: %.900s", log.c_str());
+            }
+            size_t cs = 0;
+            hiprtcGetCodeSize(prog, &cs);
+            std::vector<char> code(cs);
+            hiprtcGetCode(prog, code.data());
+            hiprtcDestroyProgram(&prog);
+            HIPCHK(hipModuleLoadData(&mod, code.data()));
+            mc.mods.emplace(std::move(key), mod);
+            if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+    }
+    p->group_fn.assign(p->h_groups.size(), nullptr);
+    p->group_grid.assign(p->h_groups.size(), 1);
+    for (size_t gi = 0; gi < p->h_groups.size(); ++gi) {
+        const std::string name = "dcs_group_" + std::to_string(gi);
+        HIPCHK(hipModuleGetFunction(&p->group_fn[gi], mod, name.c_str()));
+        int nb = 0;
+        HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, p->group_fn[gi], kThreads, 0));
+        const u64 cap = (u64)p->ctx->cus * (u64)std::max(1, nb);
+        p->group_grid[gi] = (unsigned)std::max<u64>(1, std::min<u64>(p->h_groups[gi].ntiles, cap));
+    }
+    p->compiled = true;
+    return DCS_OK;
+}
+
+template <class S>
+int launch_compiled(dcs_plan *p, int nl, const Seg &sg, int nunits) {
+    S *E = (S *)p->frontier;
+    u64 ls = p->lane_stride;
+    const int *lanes = p->d_lanes;
+    const Dyn *dyn = p->d_dyn;
+    int nsteps = p->nsteps, nu = nunits, unit = sg.unit;
+    u64 *dmax = p->d_dmax;
+    void *args[] = {&E, &ls, &lanes, &dyn, &nsteps, &dmax, &nu, &unit};
+    PROF(2, HIPCHK(hipModuleLaunchKernel(p->group_fn[sg.group], p->group_grid[sg.group], (unsigned)nl, 1, kThreads, 1, 1, 0,
+                                         p->ctx->stream, args, nullptr)));
+    return DCS_OK;
+}
+
 template <class S>
 int run_contract(dcs_plan *p, int mode, int nl, std::vector<u64> &ntiles_of_group) {
-    const std::vector<Seg> &sched = (mode == DCS_MODE_FUSED) ? p->sched_fused : p->sched_steps;
-    const int nunits = (mode == DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
+    const bool fused = mode == DCS_MODE_FUSED || mode == DCS_MODE_COMPILED;
+    const std::vector<Seg> &sched = fused ? p->sched_fused : p->sched_steps;
+    const int nunits = fused ? p->units_fused : p->units_steps;
     {
         const u64 cnt = (u64)nl * nunits;
         k_fill_u64<<<grid_for(cnt, p->ctx->cus, 4), kThreads, 0, p->ctx->stream>>>(p->d_dmax, cnt, 0ull);
@@ -1015,6 +1100,7 @@ int run_contract(dcs_plan *p, int mode, int nl, std::vector<u64> &ntiles_of_grou
         int rc = DCS_OK;
         if (sg.type == 0) rc = launch_step<S>(p, nl, sg.first, nunits, sg.unit);
         else if (sg.type == 1) rc = launch_resident<S>(p, nl, sg, nunits);
+        else if (mode == DCS_MODE_COMPILED) rc = launch_compiled<S>(p, nl, sg, nunits);
         else rc = launch_coset<S>(p, nl, sg, nunits, ntiles_of_group[sg.group]);
         if (rc) return rc;
         if (mode == DCS_MODE_STEPS_HOST) {
@@ -1149,6 +1235,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     p->tile_log2 = d->tile_log2;
     p->exact_rounding = d->exact_rounding;
     p->pass_max_dim = d->pass_max_dim;
+    p->lds_storage = d->lds_storage ? 1 : 0;
     p->esz = d->storage == DCS_STORAGE_C64 ? sizeof(float2) : sizeof(double2);
     p->lane_stride = 1ull << d->maxslots;
     p->terminal_mask = d->terminal_mask;
@@ -1278,23 +1365,32 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(DCS_ERR_RUNTIME, "plan upload sync failed"));
     p->group_ntiles.resize(groups.size());
     for (size_t i = 0; i < groups.size(); ++i) p->group_ntiles[i] = groups[i].ntiles;
+    p->h_groups = std::move(groups);
+    p->h_passes = std::move(passes);
+    p->h_psteps = std::move(psteps);
+    p->h_addrv = std::move(addrv);
     *out = p;
     return DCS_OK;
 }
 
 
+int dcs_plan_compile(dcs_plan *p, double *seconds) {
+    if (!p) return fail(DCS_ERR_INVALID, "null plan");
+    return compile_groups(p, seconds);
+}
+
 int dcs_plan_units(dcs_plan *p, int mode, int *nunits) {
     if (!p || !nunits) return fail(DCS_ERR_INVALID, "null argument");
-    if (mode < 0 || mode > 2) return fail(DCS_ERR_INVALID, "mode %d", mode);
-    *nunits = (mode == DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
+    if (mode < 0 || mode > 3) return fail(DCS_ERR_INVALID, "mode %d", mode);
+    *nunits = (mode >= DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
     return DCS_OK;
 }
 
 int dcs_plan_schedule(dcs_plan *p, int mode, int cap, int32_t *seg_kind, int32_t *first, int32_t *count,
                       int32_t *dim, int *nseg) {
     if (!p || !nseg) return fail(DCS_ERR_INVALID, "null argument");
-    if (mode < 0 || mode > 2) return fail(DCS_ERR_INVALID, "mode %d", mode);
-    const std::vector<Seg> &s = (mode == DCS_MODE_FUSED) ? p->sched_fused : p->sched_steps;
+    if (mode < 0 || mode > 3) return fail(DCS_ERR_INVALID, "mode %d", mode);
+    const std::vector<Seg> &s = (mode >= DCS_MODE_FUSED) ? p->sched_fused : p->sched_steps;
     *nseg = (int)s.size();
     for (int i = 0; i < (int)s.size() && i < cap; ++i) {
         if (seg_kind) seg_kind[i] = s[i].type;
@@ -1308,7 +1404,7 @@ int dcs_plan_schedule(dcs_plan *p, int mode, int cap, int32_t *seg_kind, int32_t
 int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const double *coeffs, const int32_t *ov,
                  const int32_t *y, double *unit_m2) {
     if (!p || !lanes || !coeffs || !ov || !y || !unit_m2) return fail(DCS_ERR_INVALID, "null argument");
-    if (mode < 0 || mode > 2) return fail(DCS_ERR_INVALID, "mode %d", mode);
+    if (mode < 0 || mode > 3) return fail(DCS_ERR_INVALID, "mode %d", mode);
     if (nl < 1 || nl > p->lanes) return fail(DCS_ERR_INVALID, "nlanes %d outside 1..%d", nl, p->lanes);
     for (int i = 0; i < nl; ++i) {
         if (lanes[i] < 0 || lanes[i] >= p->lanes) return fail(DCS_ERR_INVALID, "lane %d out of range", lanes[i]);
@@ -1316,6 +1412,10 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
             if (lanes[j] == lanes[i]) return fail(DCS_ERR_INVALID, "lane %d repeated", lanes[i]);
     }
     HIPCHK(hipSetDevice(p->ctx->device));
+    if (mode == DCS_MODE_COMPILED && !p->compiled) {
+        const int rc = compile_groups(p, nullptr);
+        if (rc) return rc;
+    }
     const int n = p->nsteps;
     for (int li = 0; li < nl; ++li)
         for (int s = 0; s < n; ++s) {
@@ -1339,7 +1439,7 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
         (void)hipStreamSynchronize(st);
         return rc;
     }
-    const int nunits = (mode == DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
+    const int nunits = (mode >= DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
     HIPCHK(hipMemcpyAsync(p->h_dmax, p->d_dmax, sizeof(u64) * (size_t)nl * nunits, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     memcpy(unit_m2, p->h_dmax, sizeof(double) * (size_t)nl * nunits);

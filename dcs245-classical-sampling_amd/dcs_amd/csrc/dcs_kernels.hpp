@@ -4,9 +4,14 @@
 // first and fuse its real part, x*y = (fma(xr, yr, -(xi*yi)), fma(xr, yi, xi*yr)); everything else
 // is unfused (the library is compiled with -ffp-contract=off).  Operand order follows the complex128
 // orbit executor kernel_opt/inplace_plan.py:97-127.
+//
+// The same text is compiled into the runtime-generated group kernels (hipRTC), which define
+// DCS_RTC and the fixed-width integer types before it.
 #pragma once
+#ifndef DCS_RTC
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 typedef unsigned long long u64;
 
@@ -59,6 +64,31 @@ __device__ __forceinline__ double scale_from_m2(double m2) {
     (void)frexp(m2, &E2);
     const int e = (m2 == 0.0) ? 0 : ((E2 + 1) >> 1);
     return ldexp(1.0, -e);
+}
+
+// Per-lane, per-step parameters of a contraction: the step vertex's local vector (f0, f1), the offset
+// parity bit and the forced value of a structurally fixed vertex.
+struct Dyn {
+    double f0r, f0i, f1r, f1i;
+    int32_t ov, y;
+    int32_t pad[2];
+};
+
+__device__ __forceinline__ double bits2d(u64 b) { return __longlong_as_double((long long)b); }
+
+// Wave-level max, then one atomic per wave (values are >= 0 or NaN, so the u64 order of the
+// float64 bit patterns is their numeric order, NaN above +inf).
+__device__ __forceinline__ void commit_max(double lm, u64 *slot) {
+    for (int off = 32; off > 0; off >>= 1) lm = max_nan(lm, __shfl_xor(lm, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(slot, (u64)__double_as_longlong(lm));
+}
+
+// Scale, round to storage, store, and fold |stored|^2 (in storage precision) into the running max.
+template <class S>
+__device__ __forceinline__ void put_scaled(S *E, u64 c, cd v, double scale, double &lm) {
+    const S o = Store<S>::round(cd{v.r * scale, v.i * scale});
+    E[c] = o;
+    lm = max_nan(lm, Store<S>::norm2(o));
 }
 
 __device__ __forceinline__ u64 pdep_loop(u64 x, u64 mask) {

@@ -1,0 +1,351 @@
+// Source generator of plan-specialised coset-group kernels (compiled at plan set-up with hipRTC).
+//
+// The generic k_coset kernel reads every group, pass and step descriptor at run time and selects
+// the orbit pattern, the role of each element and the uniform part of every sign with scalar
+// branches.  For a fixed plan all of these are constants, so this generator writes one kernel per
+// fused group in which they are literals: each step becomes straight-line FP64 code on registers,
+// signs become negations folded into the arithmetic, and only the per-slice parameters (f0, f1, the
+// offset bit ov, the forced value y) and the lane-dependent part of the sign stay dynamic.  The
+// per-step formulas are those of dcs_kernels.hpp (orb_*), with the same operand order, so a step's
+// result is bit-identical to the per-step kernel's before rounding to storage.
+//
+// Pipeline of one workgroup (persistent over the tiles of its group):
+//   store the previous tile's results (LDS -> HBM), issue the next tile's loads (HBM -> registers),
+//   run the register passes on the current tile (LDS), then move the loaded tile into the other LDS
+//   buffer.  Stores and loads are in flight during the passes; the only wait on global memory comes
+//   after them.
+//
+// Host-side code (included by dcs_amd.hip after the descriptor types).
+#pragma once
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+struct CodegenOptions {
+    int storage;       // DCS_STORAGE_C64 / DCS_STORAGE_C128
+    int tile_log2;     // log2 of the largest tile (entries)
+    bool exact;        // round to storage precision after every step
+    bool lds_storage;  // LDS tiles in storage precision (rounded once per register pass) instead of FP64
+};
+
+namespace codegen {
+
+inline void emitf(std::string &o, const char *fmt, ...) {
+    char buf[4096];
+    va_list ap;
+    va_start(ap, fmt);
+    const int n = vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (n >= (int)sizeof(buf)) {
+        std::vector<char> big(n + 1);
+        va_start(ap, fmt);
+        vsnprintf(big.data(), big.size(), fmt, ap);
+        va_end(ap);
+        o.append(big.data(), n);
+    } else {
+        o.append(buf, n);
+    }
+}
+
+inline int par32(uint32_t x) { return __builtin_popcount(x) & 1; }
+inline int top32(uint32_t x) { return 31 - __builtin_clz(x); }
+
+// Device-side definitions shared by every generated kernel of one module.
+inline std::string prelude(const CodegenOptions &opt, const char *kernels_hpp) {
+    std::string o;
+    o += "#define DCS_RTC 1\n"
+         "typedef unsigned int uint32_t;\ntypedef int int32_t;\n"
+         "typedef unsigned long long uint64_t;\ntypedef long long int64_t;\n";
+    o += kernels_hpp;
+    o += "\n";
+    const char *st = opt.storage == 0 ? "float2" : "double2";
+    const char *lt = (opt.lds_storage || opt.storage == 1) ? st : "double2";
+    emitf(o, "typedef %s St;\ntypedef %s Lt;\n", st, lt);
+    o += R"(
+__device__ __forceinline__ cd cneg(cd a) { return cd{-a.r, -a.i}; }
+template <int U> __device__ __forceinline__ cd sgn(cd a) { if constexpr (U) return cneg(a); else return a; }
+// (-1)^UA a + (-1)^UB b, evaluated as the per-element sign formulas do (negation is exact)
+template <int UA, int UB> __device__ __forceinline__ cd psum(cd a, cd b) {
+    if constexpr (!UA && !UB) return cadd(a, b);
+    else if constexpr (!UA && UB) return csub(a, b);
+    else if constexpr (UA && !UB) return csub(b, a);
+    else return cneg(cadd(a, b));
+}
+__device__ __forceinline__ cd LG(Lt v) { return Store<Lt>::load(v); }
+__device__ __forceinline__ Lt LP(cd v) { return Store<Lt>::round(v); }
+__device__ __forceinline__ cd RS(cd v) { return Store<St>::load(Store<St>::round(v)); }
+)";
+    return o;
+}
+
+// Bit deposit of a run-time value into the set bits of `mask` (lowest `nbits` bits of the value).
+inline std::string deposit_expr(const char *x, u64 mask, int nbits) {
+    std::string e = "(0ull";
+    int j = 0;
+    for (int p = 0; p < 64 && j < nbits; ++p)
+        if ((mask >> p) & 1) {
+            char buf[96];
+            snprintf(buf, sizeof(buf), " | ((((u64)(%s) >> %d) & 1ull) << %d)", x, j, p);
+            e += buf;
+            ++j;
+        }
+    return e + ")";
+}
+
+struct GroupView {
+    const GroupDev *g;
+    const PassDev *passes;   // g->npasses entries
+    const PStepDev *psteps;  // indexed by PassDev::first_pstep
+    const u64 *addrv;        // 2^d entries: physical offset of coset coordinate t
+};
+
+inline u64 addr_of(const GroupView &v, uint32_t t) { return v.addrv[t]; }
+
+// Per-element lane deposit (constant part): pdep of a constant lane coordinate.
+inline u64 lane_dep_const(const GroupDev &g, uint32_t l) {
+    if (g.lane_identity) return l;
+    u64 r = 0;
+    int j = 0;
+    for (int p = 0; p < 64 && j < g.wbits; ++p)
+        if ((g.rep_low_mask >> p) & 1) {
+            if ((l >> j) & 1) r |= 1ull << p;
+            ++j;
+        }
+    return r;
+}
+
+// One register pass.  `T` names the current LDS buffer pointer in the generated code.
+inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView &v, const PassDev &ps, int threads) {
+    const GroupDev &g = *v.g;
+    const int J = ps.nmask, NE = 1 << J, WB = g.wbits, D = g.d;
+    const uint32_t norb = (1u << (D - J)) << WB;
+    const int OPT = norb >= (uint32_t)threads ? (int)(norb / threads) : 1;
+    uint32_t TE[16];
+    for (int E = 0; E < NE; ++E) {
+        uint32_t t = 0;
+        for (int l = 0; l < J; ++l)
+            if ((E >> l) & 1) t ^= ps.mw[l];
+        TE[E] = t;
+    }
+    emitf(o, "    { // pass: span %d, %d steps\n", J, ps.nsteps);
+    if (norb < (uint32_t)threads) emitf(o, "    if (tid < %uu) {\n", norb);
+    for (int m = 0; m < OPT; ++m) {
+        emitf(o, "    const uint32_t o%d = tid + %uu;\n", m, (uint32_t)(m * threads));
+        emitf(o, "    const uint32_t l%d = o%d & %uu;\n", m, m, (1u << WB) - 1);
+        emitf(o, "    uint32_t t%d = o%d >> %d;\n", m, m, WB);
+        for (int i = 0; i < J; ++i) emitf(o, "    t%d = insert0(t%d, %d);\n", m, m, ps.piv[i]);
+        emitf(o, "    const uint32_t b%d = (t%d << %d) | l%d;\n", m, m, WB, m);
+        for (int E = 0; E < NE; ++E) emitf(o, "    cd x%d_%d = LG(T[b%d ^ %uu]);\n", m, E, m, TE[E] << WB);
+    }
+    for (int s = 0; s < ps.nsteps; ++s) {
+        const PStepDev &q = v.psteps[ps.first_pstep + s];
+        emitf(o, "    { // step %d kind %d\n", q.step, q.kind);
+        emitf(o, "      const Dyn &q = dl[%d];\n", q.step);
+        emitf(o, "      const cd f0{q.f0r, q.f0i};\n");
+        emitf(o, "      const int sg = (__popcll(tb & 0x%llxull) + q.ov) & 1;\n", (unsigned long long)q.beta);
+        auto u = [&](int E) { return par32(TE[E] & q.bvec); };
+        for (int m = 0; m < OPT; ++m) {
+            emitf(o, "      const int sb%d = (__popc(t%d & %uu) + __popc(l%d & %uu) + sg) & 1;\n", m, m, q.bvec, m, q.beta_lane);
+            emitf(o, "      const cd F%d = sb%d ? cd{-q.f1r, -q.f1i} : cd{q.f1r, q.f1i};\n", m, m);
+            if (q.kind == 2 || q.kind == 7)
+                emitf(o, "      const cd W%d_0 = cadd(f0, F%d), W%d_1 = csub(f0, F%d);\n", m, m, m, m);
+            if (q.kind == 4) emitf(o, "      const cd C%d_0 = q.y ? F%d : f0, C%d_1 = q.y ? cneg(F%d) : f0;\n", m, m, m, m);
+        }
+        for (int m = 0; m < OPT; ++m) {
+            if (q.kind == 2) {
+                for (int E = 0; E < NE; ++E) emitf(o, "      x%d_%d = cmul(x%d_%d, W%d_%d);\n", m, E, m, E, m, u(E));
+            } else if (q.kind == 4) {
+                for (int E = 0; E < NE; ++E) emitf(o, "      x%d_%d = cmul(x%d_%d, C%d_%d);\n", m, E, m, E, m, u(E));
+            } else if (q.kind == 6) {
+                const uint32_t b1 = q.pat, b2 = q.pat2;
+                for (int E = 0; E < NE; ++E) {
+                    if ((E & b1) || (E & b2)) continue;
+                    int c = -1;
+                    const int mem[4] = {E, (int)(E ^ b1), (int)(E ^ b2), (int)(E ^ b1 ^ b2)};
+                    for (int X : mem)
+                        if (!par32(X & q.pm) && !par32(X & q.pma)) c = X;
+                    const int c1 = c ^ b1, dd = c ^ b2, d1 = dd ^ b1;
+                    emitf(o, "      { const cd a = x%d_%d, b = x%d_%d, e = x%d_%d, g = x%d_%d;\n", m, c, m, c1, m, dd, m, d1);
+                    emitf(o, "        x%d_%d = cadd(cmul(f0, cadd(a, b)), cmul(F%d, psum<%d, %d>(e, g)));\n", m, c, m, u(dd), u(d1));
+                    emitf(o, "        x%d_%d = cadd(cmul(f0, cadd(e, g)), cmul(F%d, psum<%d, %d>(a, b))); }\n", m, dd, m, u(c), u(c1));
+                }
+            } else {
+                const uint32_t PI = q.pat;
+                const int tp = top32(PI);
+                for (int E = 0; E < NE; ++E) {
+                    if ((E >> tp) & 1) continue;
+                    const int EP = E ^ PI;
+                    const int A = par32(E & q.pm) ? EP : E, B = A ^ PI;
+                    switch (q.kind) {
+                        case 0:
+                            emitf(o, "      { const cd a = x%d_%d; x%d_%d = cmul(f0, a); x%d_%d = sgn<%d>(cmul(F%d, a)); }\n", m, A, m,
+                                  A, m, B, u(A), m);
+                            break;
+                        case 1:
+                            emitf(o, "      { const cd a = x%d_%d, b = x%d_%d;\n", m, A, m, B);
+                            emitf(o, "        x%d_%d = cadd(cmul(f0, a), sgn<%d>(cmul(F%d, b)));\n", m, A, u(B), m);
+                            emitf(o, "        x%d_%d = cadd(cmul(f0, b), sgn<%d>(cmul(F%d, a))); }\n", m, B, u(A), m);
+                            break;
+                        case 3:
+                            emitf(o, "      { const cd a = x%d_%d, b = x%d_%d;\n", m, A, m, B);
+                            emitf(o, "        x%d_%d = cmul(f0, cadd(a, b)); x%d_%d = cmul(F%d, psum<%d, %d>(a, b)); }\n", m, A, m,
+                                  B, m, u(A), u(B));
+                            break;
+                        case 5:
+                            emitf(o, "      { const cd a = x%d_%d, b = x%d_%d;\n", m, A, m, B);
+                            emitf(o, "        x%d_%d = q.y ? cmul(F%d, psum<%d, %d>(a, b)) : cmul(f0, cadd(a, b)); }\n", m, A, m, u(A),
+                                  u(B));
+                            break;
+                        default:  // 7
+                            emitf(o, "      x%d_%d = cadd(cmul(W%d_%d, x%d_%d), cmul(W%d_%d, x%d_%d));\n", m, A, m, u(A), m, A, m,
+                                  u(B), m, B);
+                            break;
+                    }
+                }
+            }
+            if (opt.exact)
+                for (int E = 0; E < NE; ++E) emitf(o, "      x%d_%d = RS(x%d_%d);\n", m, E, m, E);
+        }
+        emitf(o, "    }\n");
+    }
+    for (int m = 0; m < OPT; ++m)
+        for (int E = 0; E < NE; ++E) emitf(o, "    T[b%d ^ %uu] = LP(x%d_%d);\n", m, TE[E] << WB, m, E);
+    if (norb < (uint32_t)threads) emitf(o, "    }\n");
+    emitf(o, "    }\n");
+}
+
+// Kernel `dcs_group_<gi>` of one fused group.
+inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupView &v) {
+    const GroupDev &g = *v.g;
+    const int threads = 256;
+    const int D = g.d, WB = g.wbits;
+    const uint32_t W = 1u << WB, N = (1u << D) << WB;
+    const int PER = N >= (uint32_t)threads ? (int)(N / threads) : 1;
+    const bool guard = N < (uint32_t)threads;
+    std::string o;
+    emitf(o, "extern \"C\" __global__ void __launch_bounds__(%d) dcs_group_%d(St *__restrict__ base, u64 lane_stride,\n", threads, gi);
+    o += "    const int *__restrict__ lane_ids, const Dyn *__restrict__ dyn, int nsteps, u64 *__restrict__ dmax, int nunits,\n"
+         "    int unit) {\n";
+    emitf(o, "  __shared__ Lt buf[%u];\n", 2 * N);
+    emitf(o, "  const u64 NT = %lluull;\n", (unsigned long long)g.ntiles);
+    o += "  u64 ti = blockIdx.x;\n  if (ti >= NT) return;\n"
+         "  const uint32_t tid = threadIdx.x;\n"
+         "  const int li = blockIdx.y;\n"
+         "  St *__restrict__ E = base + (u64)lane_ids[li] * lane_stride;\n"
+         "  const Dyn *__restrict__ dl = dyn + (u64)li * nsteps;\n"
+         "  const double scale = scale_from_m2(unit ? bits2d(dmax[(u64)li * nunits + unit - 1]) : 1.0);\n"
+         "  double lm = 0.0;\n";
+    // Thread part of the element addresses and of the dead masks.  Element e = tid + 256 k has
+    // t = e >> WB and l = e & (W - 1); deposit and addrv are linear, so the address splits into a
+    // per-thread part and a per-k constant.
+    std::vector<u64> Ck(PER);
+    std::vector<int> din_k(PER), dout_k(PER);
+    if (WB >= 8 || guard) {
+        // l holds all of tid (WB >= 8), or the tile is smaller than the workgroup
+        const int tid_lane_bits = std::min(WB, 8);
+        const int tid_t_bits = guard ? std::max(0, 8 - WB) : 0;
+        std::string lp = g.lane_identity ? std::string("(u64)(tid & ") + std::to_string((1u << tid_lane_bits) - 1) + "u)"
+                                         : deposit_expr("tid", g.rep_low_mask, tid_lane_bits);
+        emitf(o, "  u64 pth = %s;\n", lp.c_str());
+        if (tid_t_bits > 0) {
+            for (int i = 0; i < std::min(tid_t_bits, D); ++i)
+                emitf(o, "  pth ^= (0ull - (u64)((tid >> %d) & 1u)) & 0x%llxull;\n", WB + i,
+                      (unsigned long long)addr_of(v, 1u << i));
+            emitf(o, "  const bool tdin = ((tid >> %d) & %uu) != 0;\n", WB, g.dead_in_t);
+            emitf(o, "  const bool tdout = ((tid >> %d) & %uu) != 0;\n", WB, g.dead_out_t);
+        } else {
+            o += "  const bool tdin = false, tdout = false;\n";
+        }
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t e0 = (uint32_t)(k * threads);
+            const uint32_t t = guard ? 0u : (e0 >> WB), l = guard ? 0u : (e0 & (W - 1));
+            Ck[k] = lane_dep_const(g, l) ^ addr_of(v, t);
+            din_k[k] = (t & g.dead_in_t) != 0;
+            dout_k[k] = (t & g.dead_out_t) != 0;
+        }
+    } else {
+        const int tb_t = 8 - WB;  // t bits carried by tid
+        std::string lp = g.lane_identity ? std::string("(u64)(tid & ") + std::to_string(W - 1) + "u)"
+                                         : deposit_expr("tid", g.rep_low_mask, WB);
+        emitf(o, "  u64 pth = %s;\n", lp.c_str());
+        for (int i = 0; i < std::min(tb_t, D); ++i)
+            emitf(o, "  pth ^= (0ull - (u64)((tid >> %d) & 1u)) & 0x%llxull;\n", WB + i, (unsigned long long)addr_of(v, 1u << i));
+        emitf(o, "  const bool tdin = ((tid >> %d) & %uu) != 0;\n", WB, g.dead_in_t & ((1u << tb_t) - 1));
+        emitf(o, "  const bool tdout = ((tid >> %d) & %uu) != 0;\n", WB, g.dead_out_t & ((1u << tb_t) - 1));
+        for (int k = 0; k < PER; ++k) {
+            const uint32_t t = (uint32_t)k << tb_t;
+            Ck[k] = addr_of(v, t);
+            din_k[k] = (t & g.dead_in_t) != 0;
+            dout_k[k] = (t & g.dead_out_t) != 0;
+        }
+    }
+    // tile base
+    {
+        std::string dep;
+        if (g.high.n < 0) {
+            dep = "pdep_loop(x, " + std::to_string((unsigned long long)g.high.mask) + "ull)";
+        } else {
+            dep = "(0ull";
+            for (int r = 0; r < g.high.n; ++r) {
+                char buf[128];
+                snprintf(buf, sizeof(buf), " | (((x >> %d) & 0x%llxull) << %d)", g.high.src[r],
+                         (unsigned long long)((g.high.len[r] >= 64) ? ~0ull : ((1ull << g.high.len[r]) - 1)), g.high.dst[r]);
+                dep += buf;
+            }
+            dep += ")";
+        }
+        emitf(o, "  auto tile_base = [](u64 x) -> u64 { return %s; };\n", dep.c_str());
+    }
+    const char *gd = guard ? " && tid < " : "";
+    std::string gexpr = guard ? (std::string(gd) + std::to_string(N) + "u") : "";
+    // loads into registers
+    emitf(o, "  St R[%d];\n", PER);
+    o += "  auto issue = [&](u64 tbx) {\n";
+    for (int k = 0; k < PER; ++k) {
+        if (din_k[k]) emitf(o, "    R[%d] = Store<St>::zero();\n", k);
+        else
+            emitf(o, "    R[%d] = (!tdin%s) ? E[tbx ^ pth ^ 0x%llxull] : Store<St>::zero();\n", k, gexpr.c_str(),
+                  (unsigned long long)Ck[k]);
+    }
+    o += "  };\n";
+    o += "  auto put = [&](Lt *P) {\n";
+    for (int k = 0; k < PER; ++k) {
+        if (guard) emitf(o, "    if (tid < %uu) P[tid] = LP(Store<St>::load(R[%d]));\n", N, k);
+        else emitf(o, "    P[tid + %u] = LP(Store<St>::load(R[%d]));\n", (uint32_t)(k * threads), k);
+    }
+    o += "  };\n";
+    o += "  auto store = [&](const Lt *P, u64 tbx) {\n";
+    for (int k = 0; k < PER; ++k) {
+        if (dout_k[k]) continue;
+        if (guard) emitf(o, "    if (!tdout && tid < %uu) put_scaled(E, tbx ^ pth ^ 0x%llxull, LG(P[tid]), scale, lm);\n", N,
+                         (unsigned long long)Ck[k]);
+        else
+            emitf(o, "    if (!tdout) put_scaled(E, tbx ^ pth ^ 0x%llxull, LG(P[tid + %u]), scale, lm);\n", (unsigned long long)Ck[k],
+                  (uint32_t)(k * threads));
+    }
+    o += "  };\n";
+    // pipeline
+    emitf(o, "  u64 tb = tile_base(ti);\n  issue(tb);\n  put(buf);\n  __syncthreads();\n");
+    emitf(o, "  int c = 0;\n  u64 tprev = 0;\n  bool have_prev = false;\n");
+    emitf(o, "  for (;;) {\n");
+    emitf(o, "    Lt *T = buf + c * %u;\n    Lt *U = buf + (c ^ 1) * %u;\n", N, N);
+    o += "    if (have_prev) store(U, tprev);\n"
+         "    const u64 tn = ti + gridDim.x;\n"
+         "    const bool more = tn < NT;\n"
+         "    if (more) issue(tile_base(tn));\n";
+    for (int pi = 0; pi < g.npasses; ++pi) {
+        emit_pass(o, opt, v, v.passes[pi], threads);
+        if (pi + 1 < g.npasses) o += "    __syncthreads();\n";
+    }
+    o += "    if (more) put(U);\n"
+         "    __syncthreads();\n"
+         "    if (!more) { store(T, tb); break; }\n"
+         "    have_prev = true;\n    tprev = tb;\n    c ^= 1;\n    ti = tn;\n    tb = tile_base(ti);\n"
+         "  }\n"
+         "  commit_max(lm, &dmax[(u64)li * nunits + unit]);\n"
+         "}\n";
+    return o;
+}
+
+}  // namespace codegen

@@ -62,7 +62,9 @@ enum { DCS_STORAGE_C64 = 0, DCS_STORAGE_C128 = 1 };
 enum {
     DCS_MODE_STEPS = 0,       /* one launch per step, scale chain kept on the device, no host sync */
     DCS_MODE_STEPS_HOST = 1,  /* one launch per step, the host reads the maximum after every step */
-    DCS_MODE_FUSED = 2        /* LDS-resident runs + fused coset groups (the production schedule) */
+    DCS_MODE_FUSED = 2,       /* LDS-resident runs + fused coset groups, generic coset kernel */
+    DCS_MODE_COMPILED = 3     /* the fused schedule with one plan-specialised kernel per coset group,
+                                 compiled at run time (dcs_plan_compile); the production schedule */
 };
 
 typedef struct {
@@ -94,6 +96,8 @@ typedef struct {
                                    mantissas equal the per-step schedule); 0: round once per register
                                    pass (more accurate, fewer conversions) */
     int32_t pass_max_dim;       /* span of the orbit masks of one register pass, 3 or 4 */
+    int32_t lds_storage;        /* DCS_MODE_COMPILED: 1 holds coset tiles in LDS in storage precision
+                                   (rounded once per register pass), 0 in FP64 */
 } dcs_plan_desc;
 
 typedef struct {
@@ -115,6 +119,11 @@ int dcs_synchronize(dcs_context *ctx);
 
 int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *desc, dcs_plan **out);
 void dcs_plan_destroy(dcs_plan *plan);
+/* Generate and compile the plan-specialised group kernels of DCS_MODE_COMPILED (hipRTC, gfx950).
+ * Compiled modules are cached per device for the life of the process, keyed by their source.
+ * *seconds (may be NULL) receives the compile time (0 when served from the cache).  dcs_contract in
+ * DCS_MODE_COMPILED calls it on first use. */
+int dcs_plan_compile(dcs_plan *plan, double *seconds);
 /* Number of scale units (maxima) a contraction in `mode` produces per lane. */
 int dcs_plan_units(dcs_plan *plan, int mode, int *nunits);
 /* Schedule of `mode`: per segment (kind 0 step, 1 resident run, 2 coset group), first step, step

@@ -31,9 +31,10 @@ def bits(s):
     return np.array([int(c) for c in s], np.uint8)
 
 
-def test_ibm64_c128_k3_matches_scores(ibm64, records):
-    print('case ibm64 c128 k3 fused', flush=True)
-    e = make(ibm64, 'inplace_k3_fp64', np.complex128, schedule='fused')
+@pytest.mark.parametrize('sched', ['fused', 'compiled'])
+def test_ibm64_c128_k3_matches_scores(ibm64, records, sched):
+    print(f'case ibm64 c128 k3 {sched}', flush=True)
+    e = make(ibm64, 'inplace_k3_fp64', np.complex128, schedule=sched)
     audited = sorted(int(k) for k in records['audits'])[:1]
     for sid in [0] + audited:
         s = records['samples'][str(sid)]
@@ -50,9 +51,11 @@ def test_ibm64_c128_k3_matches_scores(ibm64, records):
     e.close()
 
 
-def test_ibm64_c64_k2_draws_match_r1(ibm64, records):
-    print('case ibm64 c64 k2 fused', flush=True)
-    e = make(ibm64, 'inplace_k2_fp32', np.complex64, schedule='fused')
+@pytest.mark.parametrize('sched,lds', [('fused', False), ('compiled', False), ('compiled', True)])
+def test_ibm64_c64_k2_draws_match_r1(ibm64, records, sched, lds):
+    print(f'case ibm64 c64 k2 {sched} lds_storage={lds}', flush=True)
+    kw = {'lds_storage': True} if lds else {}
+    e = make(ibm64, 'inplace_k2_fp32', np.complex64, schedule=sched, **kw)
     ties = 0
     phase = None
     for sid in range(3):

@@ -62,25 +62,63 @@ def test_c1_c64_steps_bit_exact(c1):
     e.close()
 
 
+@pytest.mark.parametrize('sched', ['fused', 'compiled'])
 @pytest.mark.parametrize('rb,dim,tl,pj', [(12, 6, 12, 3), (6, 6, 12, 4), (6, 8, 11, 3), (0, 4, 11, 4), (6, 1, 12, 3)])
-def test_c1_fused_matches_steps(c1, rb, dim, tl, pj):
+def test_c1_fused_matches_steps(c1, rb, dim, tl, pj, sched):
     """complex128 fused and complex64 fused with per-step rounding keep the per-step mantissas
     (differences only through exp(ls)); complex64 fused with per-pass rounding stays within 1e-5 of
     FP64 and no further from it than the per-step complex64 schedule (x2 margin)."""
-    print(f'case c1 fused rb={rb} dim={dim} tl={tl}', flush=True)
+    print(f'case c1 {sched} rb={rb} dim={dim} tl={tl} pj={pj}', flush=True)
     for dt, ref, tol, exact in ((np.complex128, c1['ref128'], 1e-13, False), (np.complex64, c1['ref64'], 1e-13, True)):
-        e = engine(c1['args'], dtype=dt, schedule='fused', resident_bits=rb, fuse_max_dim=dim, tile_log2=tl,
+        e = engine(c1['args'], dtype=dt, schedule=sched, resident_bits=rb, fuse_max_dim=dim, tile_log2=tl,
                    exact_rounding=exact, pass_max_dim=pj)
         v, _ = e.block(c1['x'])
         assert rel(v, ref) < tol, (dt, rel(v, ref))
         e.close()
-    e = engine(c1['args'], dtype=np.complex64, schedule='fused', resident_bits=rb, fuse_max_dim=dim, tile_log2=tl,
+    e = engine(c1['args'], dtype=np.complex64, schedule=sched, resident_bits=rb, fuse_max_dim=dim, tile_log2=tl,
                pass_max_dim=pj)
     v, _ = e.block(c1['x'])
     err, err_steps = rel(v, c1['ref128']), rel(c1['ref64'], c1['ref128'])
     print(f'c64 fused error {err:.3e}, per-step c64 error {err_steps:.3e}', flush=True)
     assert err < 1e-5 and err <= 2 * err_steps
     e.close()
+
+
+@pytest.fixture(scope='module')
+def w20():
+    """28 qubits, depth 32, 64 T: frontier width 20, so coset groups run on full 2^11 / 2^12 tiles."""
+    inst = synthetic_instance(28, 32, 64, seed=0)
+    args = (inst['graph'], inst['unitaries'], 28, inst['natural_order'], [], list(range(20, 28)))
+    bp = BlockPlan(*args)
+    assert bp.inplace_maxslots == 20
+    x = np.random.default_rng(11).integers(0, 2, 28, dtype=np.uint8)
+    return {'args': args, 'x': x, 'ref128': oracle.block(bp, x), 'ref64': oracle.block(bp, x, storage=0)}
+
+
+@pytest.mark.parametrize('tl,pj,lds', [(11, 3, False), (11, 3, True), (12, 4, True), (12, 3, False)])
+def test_w20_compiled_matches_oracle(w20, tl, pj, lds):
+    """Generated group kernels: complex128 within 1e-13 of the oracle; with per-step rounding the
+    complex64 block equals the generic coset kernel's bit for bit; the production complex64 setting
+    stays within 1e-5 of FP64 and no further than the per-step complex64 schedule (x2 margin)."""
+    print(f'case w20 compiled tl={tl} pj={pj} lds={lds}', flush=True)
+    e = engine(w20['args'], dtype=np.complex128, schedule='compiled', tile_log2=tl, pass_max_dim=pj, lds_storage=lds)
+    v = np.asarray(e.block(w20['x'])[0])
+    e.close()
+    assert rel(v, w20['ref128']) < 1e-13, rel(v, w20['ref128'])
+    blocks = []
+    for sched in ('fused', 'compiled'):
+        e = engine(w20['args'], dtype=np.complex64, schedule=sched, tile_log2=tl, pass_max_dim=pj, exact_rounding=True,
+                   lds_storage=lds)
+        blocks.append(np.asarray(e.block(w20['x'])[0]))
+        e.close()
+    assert np.array_equal(blocks[0], blocks[1])
+    assert rel(blocks[1], w20['ref64']) < 1e-13
+    e = engine(w20['args'], dtype=np.complex64, schedule='compiled', tile_log2=tl, pass_max_dim=pj, lds_storage=lds)
+    v = np.asarray(e.block(w20['x'])[0])
+    e.close()
+    err, err_steps = rel(v, w20['ref128']), rel(w20['ref64'], w20['ref128'])
+    print(f'c64 compiled error {err:.3e}, per-step c64 error {err_steps:.3e}', flush=True)
+    assert err < 1e-5 and err <= 2 * err_steps
 
 
 def test_c1_matches_statevector_and_64_draws_seed0(c1):
@@ -119,10 +157,11 @@ def test_prefix_blocks_and_lanes():
     for x, r in zip(xs, refs):
         assert np.array_equal(np.asarray(e1.block(x)[0]), r)
     e1.close()
-    e3 = engine(args, dtype=np.complex64, schedule='fused', lanes=3, resident_bits=6)
-    for x, r in zip(xs, refs):
-        assert rel(e3.block(x)[0], r) < 1e-5
-    e3.close()
+    for sched in ('fused', 'compiled'):
+        e3 = engine(args, dtype=np.complex64, schedule=sched, lanes=3, resident_bits=6)
+        for x, r in zip(xs, refs):
+            assert rel(e3.block(x)[0], r) < 1e-5
+        e3.close()
 
 
 def test_parity_sliced_ring_circuit():
