@@ -7,12 +7,13 @@ every slice contraction, the coherent slice sum, the Walsh-Hadamard / phase / Cl
 device CDF draw.  Inputs (graph, plan, frontier buffer) are resident before the timed region.
 
 Fast path (value): complex64 storage, plan inplace_k2_fp32 (4 slices, rmax 34, 128 GiB frontier),
-fused schedule (LDS-resident runs + coset groups).  Plain path (reported beside it): complex128
+compiled schedule (LDS-resident runs + one generated kernel per coset group, complex64 LDS tiles).
+Plain path (reported beside it): complex128
 storage, plan inplace_k3_fp64 (8 slices, rmax 33), one launch per step and a host read of every
 maximum, one prefix at a time -- the reference method.
 
-The line also carries the roofline of the dominant kernel (k_coset, HBM-bound) measured with HIP
-events on the library's stream, and a CPU baseline: the C oracle (OpenMP) on a bounded sample of the
+The line also carries the roofline of the dominant kernels (the coset-group kernels, HBM-bound)
+measured with HIP events on the library's stream, and a CPU baseline: the C oracle (OpenMP) on a bounded sample of the
 same workload (the first steps of one slice, widths <= 2^28), extrapolated by bytes.
 """
 import argparse
@@ -58,14 +59,14 @@ def fused_bytes_per_sample(engine, elem):
     return {k: v * nslices for k, v in out.items()}
 
 
-def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False):
+def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False, **alloc_kw):
     from dcs_amd.conditional_block import InplaceConditionalBlockGPU
     from dcs_amd.workloads import load_ibm64
     w = load_ibm64()
     e = InplaceConditionalBlockGPU(w['graph'], w['unitaries'], 64, w['order'], w['plans'][plan_name]['operations'],
                                    w['open_vertices'])
     t0 = time.perf_counter()
-    e.alloc(dtype=dtype, schedule=schedule)
+    e.alloc(dtype=dtype, schedule=schedule, **alloc_kw)
     setup = time.perf_counter() - t0
     work = prefixes(warmup + nsamples)
     for x, u in work[:warmup]:
@@ -141,7 +142,8 @@ def main():
     ap.add_argument('--steps', type=int, default=4)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--plain-samples', type=int, default=1)
-    ap.add_argument('--schedule', default='fused')
+    ap.add_argument('--schedule', default='compiled', choices=['compiled', 'fused', 'steps', 'steps_host'])
+    ap.add_argument('--lds-fp64', action='store_true', help='compiled schedule: FP64 LDS tiles')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plain', action='store_true')
     a = ap.parse_args()
@@ -162,7 +164,8 @@ def main():
     log(f'stream copy: {stream_GBps:.1f} GB/s')
 
     log('fast path: complex64, k2 plan, schedule', a.schedule)
-    fast = run_path('inplace_k2_fp32', np.complex64, a.schedule, a.steps, a.warmup, profile=True)
+    kw = {'lds_storage': not a.lds_fp64} if a.schedule == 'compiled' else {}
+    fast = run_path('inplace_k2_fp32', np.complex64, a.schedule, a.steps, a.warmup, profile=True, **kw)
     eng = fast['engine']
     ms, launches = fast['profile']
     sps = a.steps / fast['elapsed']
@@ -197,7 +200,9 @@ def main():
         log('cpu', cpu)
 
     traffic = None
-    tf = ROOT / 'profiles' / 'k_coset_ibm64_k2_c64_traffic.json'
+    kernel = 'dcs_group_*' if a.schedule == 'compiled' else 'k_coset'
+    tf = ROOT / 'profiles' / ('coset_groups_ibm64_k2_c64_traffic.json' if a.schedule == 'compiled'
+                              else 'k_coset_ibm64_k2_c64_traffic.json')
     if tf.exists():
         traffic = json.loads(tf.read_text()).get('hbm_bytes_per_launch')
     lat = np.array(fast['lat']) * 1e3
@@ -210,7 +215,9 @@ def main():
         'dtype': 'c64 storage / f64 arithmetic', 'data': 'frozen IBM QAT #245 graph, protocol-replayed prefixes',
         'config': {'workload': 'IBM QAT #245 64q/d73/314T injected graph (N=378), plan inplace_k2_fp32 '
                                '(4 parity slices, rmax 34, 128 GiB c64 frontier), 2^20 block, 44-bit prefixes',
-                   'schedule': a.schedule, 'block': 1 << 20},
+                   'schedule': a.schedule, 'lds_tiles': ('fp64' if a.lds_fp64 else 'storage') if a.schedule == 'compiled' else None,
+                   'block': 1 << 20},
+        'setup_s': fast['setup'],
         'effective_hbm_GBps': unfused_bytes * sps / 1e9,
         'effective_roofline_frac': unfused_bytes * sps / HBM_PEAK,
         'algorithmic_bytes_per_sample': {'unfused': unfused_bytes, 'fused': sum(fb.values())},
@@ -220,7 +227,7 @@ def main():
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9 if achieved else None, 'peak': HBM_PEAK / 1e9,
                      'unit': 'GB/s', 'frac': (achieved / HBM_PEAK) if achieved else None, 'traffic': traffic,
                      'launches_per_sample': int(launches[2]),
-                     'kernel': 'k_coset', 'bytes_per_sample': coset_bytes, 'ms_per_sample': coset_ms},
+                     'kernel': kernel, 'bytes_per_sample': coset_bytes, 'ms_per_sample': coset_ms},
         'cpu_baseline': cpu,
         'device': {k: dev[k] for k in ('device_name', 'arch', 'compute_units', 'total_bytes')},
     }

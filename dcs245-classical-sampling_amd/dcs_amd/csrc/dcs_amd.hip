@@ -61,7 +61,8 @@ int fail(int code, const char *fmt, ...) {
 
 constexpr int kThreads = 256;
 constexpr int kMaxRuns = 16;
-constexpr int kMaxDim = 8;
+constexpr int kMaxDim = 8;          // coset dimension of the generic k_coset kernel
+constexpr int kMaxDimCompiled = 12; // coset dimension of generated group kernels
 
 struct Runs {
     int n;  // number of runs; -1: use pdep over `mask`
@@ -693,7 +694,7 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
 
     p->sched_fused.clear();
     int unit = 0;
-    const int D = std::max(1, std::min(kMaxDim, p->fuse_max_dim));
+    const int D = std::max(1, std::min(kMaxDimCompiled, p->fuse_max_dim));
     const int TL = p->tile_log2;
     int s = 0;
     while (s < n) {
@@ -976,6 +977,9 @@ int launch_resident(dcs_plan *p, int nl, const Seg &sg, int nunits) {
 
 template <class S>
 int launch_coset(dcs_plan *p, int nl, const Seg &sg, int nunits, u64 ntiles) {
+    if (sg.dim > kMaxDim)
+        return fail(DCS_ERR_INVALID, "coset dimension %d exceeds %d: use DCS_MODE_COMPILED or a smaller fuse_max_dim", sg.dim,
+                    kMaxDim);
     S *E = (S *)p->frontier;
     dim3 grid((unsigned)std::max<u64>(1, std::min<u64>(ntiles, (u64)p->ctx->cus * 8)), nl);
 #define C(T, X, JM)                                                                                     \
@@ -1216,7 +1220,8 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     if (d->tile_log2 < 11 || d->tile_log2 > 12) return fail(DCS_ERR_INVALID, "tile_log2 must be 11 or 12");
     if (d->resident_bits != 0 && (d->resident_bits < 6 || d->resident_bits > 12))
         return fail(DCS_ERR_INVALID, "resident_bits must be 0 or 6..12");
-    if (d->fuse_max_dim < 0 || d->fuse_max_dim > kMaxDim) return fail(DCS_ERR_INVALID, "fuse_max_dim must be 0..8");
+    if (d->fuse_max_dim < 0 || d->fuse_max_dim > kMaxDimCompiled)
+        return fail(DCS_ERR_INVALID, "fuse_max_dim must be 0..%d (0..%d for DCS_MODE_FUSED)", kMaxDimCompiled, kMaxDim);
     if (d->pass_max_dim < 3 || d->pass_max_dim > kPassMax) return fail(DCS_ERR_INVALID, "pass_max_dim must be 3 or 4");
     const u64 lim = (d->maxslots == 0) ? 1ull : ((1ull << d->maxslots) - 1);
     if (d->terminal_mask & ~lim) return fail(DCS_ERR_INVALID, "terminal slots beyond maxslots");

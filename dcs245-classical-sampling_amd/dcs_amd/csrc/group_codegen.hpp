@@ -75,6 +75,11 @@ template <int UA, int UB> __device__ __forceinline__ cd psum(cd a, cd b) {
 __device__ __forceinline__ cd LG(Lt v) { return Store<Lt>::load(v); }
 __device__ __forceinline__ Lt LP(cd v) { return Store<Lt>::round(v); }
 __device__ __forceinline__ cd RS(cd v) { return Store<St>::load(Store<St>::round(v)); }
+__device__ __forceinline__ float max_nan_f(float a, float b) {
+    if (a != a) return a;
+    if (b != b) return b;
+    return a > b ? a : b;
+}
 )";
     return o;
 }
@@ -216,12 +221,21 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
 }
 
 // Kernel `dcs_group_<gi>` of one fused group.
+//
+// Load / store mapping: thread tid owns elements e = tid + 256 k of every tile (element e = t * W + l
+// sits at LDS index e).  Deposit and addrv are linear, so an element's address splits into a
+// per-thread part (pth, computed once) and a per-k constant.  (Pairing adjacent lanes into 16-byte
+// accesses measured slower on the 64-qubit plan and is not used.)
 inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupView &v) {
     const GroupDev &g = *v.g;
     const int threads = 256;
     const int D = g.d, WB = g.wbits;
     const uint32_t W = 1u << WB, N = (1u << D) << WB;
-    const int PER = N >= (uint32_t)threads ? (int)(N / threads) : 1;
+    const bool c64 = opt.storage == 0;
+    const bool lds_is_storage = opt.lds_storage || !c64;  // Lt == St
+    const int EB = 8;  // bits of the per-thread element offset tid
+    const uint32_t span = (uint32_t)threads;
+    const int PER = N >= span ? (int)(N / span) : 1;
     const bool guard = N < (uint32_t)threads;
     std::string o;
     emitf(o, "extern \"C\" __global__ void __launch_bounds__(%d) dcs_group_%d(St *__restrict__ base, u64 lane_stride,\n", threads, gi);
@@ -234,45 +248,46 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
          "  const int li = blockIdx.y;\n"
          "  St *__restrict__ E = base + (u64)lane_ids[li] * lane_stride;\n"
          "  const Dyn *__restrict__ dl = dyn + (u64)li * nsteps;\n"
-         "  const double scale = scale_from_m2(unit ? bits2d(dmax[(u64)li * nunits + unit - 1]) : 1.0);\n"
-         "  double lm = 0.0;\n";
-    // Thread part of the element addresses and of the dead masks.  Element e = tid + 256 k has
-    // t = e >> WB and l = e & (W - 1); deposit and addrv are linear, so the address splits into a
-    // per-thread part and a per-k constant.
+         "  const double scale = scale_from_m2(unit ? bits2d(dmax[(u64)li * nunits + unit - 1]) : 1.0);\n";
+    if (c64 && lds_is_storage)
+        o += "  const float sf = (float)scale;\n"
+             "  const bool sfast = scale >= 0x1p-120 && scale <= 0x1p120;  // float product exact and equal\n"
+             "  float lmf = 0.0f;\n";
+    else
+        o += "  double lm = 0.0;\n";
+    o += "  const uint32_t vt = tid;\n";
     std::vector<u64> Ck(PER);
     std::vector<int> din_k(PER), dout_k(PER);
-    if (WB >= 8 || guard) {
-        // l holds all of tid (WB >= 8), or the tile is smaller than the workgroup
-        const int tid_lane_bits = std::min(WB, 8);
-        const int tid_t_bits = guard ? std::max(0, 8 - WB) : 0;
-        std::string lp = g.lane_identity ? std::string("(u64)(tid & ") + std::to_string((1u << tid_lane_bits) - 1) + "u)"
-                                         : deposit_expr("tid", g.rep_low_mask, tid_lane_bits);
+    if (WB >= EB || guard) {
+        const int vt_lane_bits = std::min(WB, EB);
+        const int vt_t_bits = guard ? std::max(0, EB - WB) : 0;
+        std::string lp = g.lane_identity ? std::string("(u64)(vt & ") + std::to_string((1u << vt_lane_bits) - 1) + "u)"
+                                         : deposit_expr("vt", g.rep_low_mask, vt_lane_bits);
         emitf(o, "  u64 pth = %s;\n", lp.c_str());
-        if (tid_t_bits > 0) {
-            for (int i = 0; i < std::min(tid_t_bits, D); ++i)
-                emitf(o, "  pth ^= (0ull - (u64)((tid >> %d) & 1u)) & 0x%llxull;\n", WB + i,
-                      (unsigned long long)addr_of(v, 1u << i));
-            emitf(o, "  const bool tdin = ((tid >> %d) & %uu) != 0;\n", WB, g.dead_in_t);
-            emitf(o, "  const bool tdout = ((tid >> %d) & %uu) != 0;\n", WB, g.dead_out_t);
+        if (vt_t_bits > 0) {
+            for (int i = 0; i < std::min(vt_t_bits, D); ++i)
+                emitf(o, "  pth ^= (0ull - (u64)((vt >> %d) & 1u)) & 0x%llxull;\n", WB + i, (unsigned long long)addr_of(v, 1u << i));
+            emitf(o, "  const bool tdin = ((vt >> %d) & %uu) != 0;\n", WB, g.dead_in_t);
+            emitf(o, "  const bool tdout = ((vt >> %d) & %uu) != 0;\n", WB, g.dead_out_t);
         } else {
             o += "  const bool tdin = false, tdout = false;\n";
         }
         for (int k = 0; k < PER; ++k) {
-            const uint32_t e0 = (uint32_t)(k * threads);
+            const uint32_t e0 = (uint32_t)k * span;
             const uint32_t t = guard ? 0u : (e0 >> WB), l = guard ? 0u : (e0 & (W - 1));
             Ck[k] = lane_dep_const(g, l) ^ addr_of(v, t);
             din_k[k] = (t & g.dead_in_t) != 0;
             dout_k[k] = (t & g.dead_out_t) != 0;
         }
     } else {
-        const int tb_t = 8 - WB;  // t bits carried by tid
-        std::string lp = g.lane_identity ? std::string("(u64)(tid & ") + std::to_string(W - 1) + "u)"
-                                         : deposit_expr("tid", g.rep_low_mask, WB);
+        const int tb_t = EB - WB;  // t bits carried by vt
+        std::string lp = g.lane_identity ? std::string("(u64)(vt & ") + std::to_string(W - 1) + "u)"
+                                         : deposit_expr("vt", g.rep_low_mask, WB);
         emitf(o, "  u64 pth = %s;\n", lp.c_str());
         for (int i = 0; i < std::min(tb_t, D); ++i)
-            emitf(o, "  pth ^= (0ull - (u64)((tid >> %d) & 1u)) & 0x%llxull;\n", WB + i, (unsigned long long)addr_of(v, 1u << i));
-        emitf(o, "  const bool tdin = ((tid >> %d) & %uu) != 0;\n", WB, g.dead_in_t & ((1u << tb_t) - 1));
-        emitf(o, "  const bool tdout = ((tid >> %d) & %uu) != 0;\n", WB, g.dead_out_t & ((1u << tb_t) - 1));
+            emitf(o, "  pth ^= (0ull - (u64)((vt >> %d) & 1u)) & 0x%llxull;\n", WB + i, (unsigned long long)addr_of(v, 1u << i));
+        emitf(o, "  const bool tdin = ((vt >> %d) & %uu) != 0;\n", WB, g.dead_in_t & ((1u << tb_t) - 1));
+        emitf(o, "  const bool tdout = ((vt >> %d) & %uu) != 0;\n", WB, g.dead_out_t & ((1u << tb_t) - 1));
         for (int k = 0; k < PER; ++k) {
             const uint32_t t = (uint32_t)k << tb_t;
             Ck[k] = addr_of(v, t);
@@ -297,8 +312,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         }
         emitf(o, "  auto tile_base = [](u64 x) -> u64 { return %s; };\n", dep.c_str());
     }
-    const char *gd = guard ? " && tid < " : "";
-    std::string gexpr = guard ? (std::string(gd) + std::to_string(N) + "u") : "";
+    const std::string gexpr = guard ? (std::string(" && tid < ") + std::to_string(N) + "u") : "";
     // loads into registers
     emitf(o, "  St R[%d];\n", PER);
     o += "  auto issue = [&](u64 tbx) {\n";
@@ -311,18 +325,31 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
     o += "  };\n";
     o += "  auto put = [&](Lt *P) {\n";
     for (int k = 0; k < PER; ++k) {
-        if (guard) emitf(o, "    if (tid < %uu) P[tid] = LP(Store<St>::load(R[%d]));\n", N, k);
-        else emitf(o, "    P[tid + %u] = LP(Store<St>::load(R[%d]));\n", (uint32_t)(k * threads), k);
+        const uint32_t off = (uint32_t)k * span;
+        if (guard) {
+            emitf(o, "    if (tid < %uu) P[tid] = LP(Store<St>::load(R[%d]));\n", N, k);
+        } else {
+            emitf(o, "    P[vt + %u] = LP(Store<St>::load(R[%d]));\n", off, k);
+        }
     }
     o += "  };\n";
     o += "  auto store = [&](const Lt *P, u64 tbx) {\n";
     for (int k = 0; k < PER; ++k) {
         if (dout_k[k]) continue;
-        if (guard) emitf(o, "    if (!tdout && tid < %uu) put_scaled(E, tbx ^ pth ^ 0x%llxull, LG(P[tid]), scale, lm);\n", N,
-                         (unsigned long long)Ck[k]);
-        else
-            emitf(o, "    if (!tdout) put_scaled(E, tbx ^ pth ^ 0x%llxull, LG(P[tid + %u]), scale, lm);\n", (unsigned long long)Ck[k],
-                  (uint32_t)(k * threads));
+        const uint32_t off = (uint32_t)k * span;
+        const unsigned long long C = (unsigned long long)Ck[k];
+        if (c64 && lds_is_storage) {
+            // complex64 values from LDS: the power-of-two scale is exact in float when representable
+            emitf(o, "    if (!tdout%s) { const float2 a = P[%s]; float2 r;\n", gexpr.c_str(),
+                  guard ? "tid" : (std::string("vt + ") + std::to_string(off)).c_str());
+            o += "      if (sfast) r = make_float2(a.x * sf, a.y * sf);\n"
+                 "      else r = make_float2((float)((double)a.x * scale), (float)((double)a.y * scale));\n";
+            emitf(o, "      E[tbx ^ pth ^ 0x%llxull] = r;\n", C);
+            o += "      lmf = max_nan_f(lmf, r.x * r.x + r.y * r.y); }\n";
+        } else {
+            emitf(o, "    if (!tdout%s) put_scaled(E, tbx ^ pth ^ 0x%llxull, LG(P[%s]), scale, lm);\n", gexpr.c_str(), C,
+                  guard ? "tid" : (std::string("vt + ") + std::to_string(off)).c_str());
+        }
     }
     o += "  };\n";
     // pipeline
@@ -342,9 +369,10 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
          "    __syncthreads();\n"
          "    if (!more) { store(T, tb); break; }\n"
          "    have_prev = true;\n    tprev = tb;\n    c ^= 1;\n    ti = tn;\n    tb = tile_base(ti);\n"
-         "  }\n"
-         "  commit_max(lm, &dmax[(u64)li * nunits + unit]);\n"
-         "}\n";
+         "  }\n";
+    if (c64 && lds_is_storage) o += "  commit_max((double)lmf, &dmax[(u64)li * nunits + unit]);\n";
+    else o += "  commit_max(lm, &dmax[(u64)li * nunits + unit]);\n";
+    o += "}\n";
     return o;
 }
 

@@ -89,7 +89,7 @@ typedef struct {
     const double *open_unitaries;   /* [k][8] u00r,u00i,u01r,u01i,u10r,u10i,u11r,u11i */
     int32_t blocks;             /* complex128 block accumulators */
     /* schedule knobs (DCS_MODE_FUSED) */
-    int32_t fuse_max_dim;       /* largest coset dimension of a fused group, 1..8 */
+    int32_t fuse_max_dim;       /* largest coset dimension of a fused group: 1..8 (DCS_MODE_FUSED), 1..12 */
     int32_t resident_bits;      /* steps whose live slots all sit below this bit run LDS-resident, <= 12 */
     int32_t tile_log2;          /* log2 entries per coset tile, 11 or 12 */
     int32_t exact_rounding;     /* 1: fused steps round to storage precision after every step (stored
