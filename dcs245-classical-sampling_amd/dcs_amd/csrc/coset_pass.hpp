@@ -34,7 +34,8 @@ struct PStepDev {
     uint32_t pat, pat2;      // local pattern of the step's mask; quad: single-bit patterns of K, Ka
     uint32_t pm, pma;        // role masks (basis vectors holding the K / Ka pivot t bit)
     uint32_t lbv;            // bit l = parity(b_l & bvec)
-    uint32_t bvec, beta_lane, pad;
+    uint32_t bvec, beta_lane;
+    uint32_t dead_t;         // t-space basis indices whose pivot slot is not live before the step
 };
 
 template <int E>

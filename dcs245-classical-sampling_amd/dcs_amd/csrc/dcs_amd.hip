@@ -727,6 +727,14 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
         u64 piv = 0;
         for (u64 v : V) piv |= 1ull << top_bit(v);
         if (piv & ~Aun) return fail(DCS_ERR_INVALID, "group pivots outside live slots at step %d", s);
+        // Coordinate order: pivots whose slot is live throughout the group first, those that die or are
+        // born inside it last, so that the latter map to the high orbit-index bits (uniform per
+        // wavefront) when a register pass does not hold them.
+        {
+            u64 always = ~0ull;
+            for (int j = s; j < e; ++j) always &= p->steps[j].in_mask & p->steps[j].out_mask;
+            std::stable_partition(V.begin(), V.end(), [&](u64 v) { return (always >> top_bit(v)) & 1; });
+        }
         const u64 R = Aun & ~piv;
         const int rbits = popc(R);
         int wb = std::min(TL - d, rbits);
@@ -924,6 +932,8 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
                         r &= r - 1;
                     }
                 }
+                for (int i = 0; i < d; ++i)
+                    if (!((st.in_mask >> top_bit(V[i])) & 1)) q.dead_t |= 1u << i;
                 psteps.push_back(q);
             }
             passes.push_back(ps);

@@ -19,6 +19,7 @@
 #pragma once
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -75,6 +76,18 @@ template <int UA, int UB> __device__ __forceinline__ cd psum(cd a, cd b) {
 __device__ __forceinline__ cd LG(Lt v) { return Store<Lt>::load(v); }
 __device__ __forceinline__ Lt LP(cd v) { return Store<Lt>::round(v); }
 __device__ __forceinline__ cd RS(cd v) { return Store<St>::load(Store<St>::round(v)); }
+// p*a + q*b, each component one chain of fused multiply-adds
+__device__ __forceinline__ cd cfma2(cd p, cd a, cd q, cd b) {
+    double r = q.r * b.r;
+    r = fma(-q.i, b.i, r);
+    r = fma(p.r, a.r, r);
+    r = fma(-p.i, a.i, r);
+    double i = q.r * b.i;
+    i = fma(q.i, b.r, i);
+    i = fma(p.r, a.i, i);
+    i = fma(p.i, a.r, i);
+    return cd{r, i};
+}
 __device__ __forceinline__ float max_nan_f(float a, float b) {
     if (a != a) return a;
     if (b != b) return b;
@@ -157,11 +170,23 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
                 emitf(o, "      const cd W%d_0 = cadd(f0, F%d), W%d_1 = csub(f0, F%d);\n", m, m, m, m);
             if (q.kind == 4) emitf(o, "      const cd C%d_0 = q.y ? F%d : f0, C%d_1 = q.y ? cneg(F%d) : f0;\n", m, m, m, m);
         }
+        // Elementary updates of this step, each keyed by the element playing "c" (an update is needed
+        // only when that element is live).  With complex64 storage outside the per-step rounding mode,
+        // two-product sums use one chain of fused multiply-adds.
+        const bool fma2 = opt.storage == 0 && !opt.exact;
         for (int m = 0; m < OPT; ++m) {
-            if (q.kind == 2) {
-                for (int E = 0; E < NE; ++E) emitf(o, "      x%d_%d = cmul(x%d_%d, W%d_%d);\n", m, E, m, E, m, u(E));
-            } else if (q.kind == 4) {
-                for (int E = 0; E < NE; ++E) emitf(o, "      x%d_%d = cmul(x%d_%d, C%d_%d);\n", m, E, m, E, m, u(E));
+            std::vector<std::pair<int, std::string>> ops;
+            char buf[512];
+            auto add = [&](int A, const char *fmt, ...) {
+                va_list ap;
+                va_start(ap, fmt);
+                vsnprintf(buf, sizeof(buf), fmt, ap);
+                va_end(ap);
+                ops.push_back({A, std::string(buf)});
+            };
+            if (q.kind == 2 || q.kind == 4) {
+                for (int E = 0; E < NE; ++E)
+                    add(E, "x%d_%d = cmul(x%d_%d, %c%d_%d);", m, E, m, E, q.kind == 2 ? 'W' : 'C', m, u(E));
             } else if (q.kind == 6) {
                 const uint32_t b1 = q.pat, b2 = q.pat2;
                 for (int E = 0; E < NE; ++E) {
@@ -171,9 +196,16 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
                     for (int X : mem)
                         if (!par32(X & q.pm) && !par32(X & q.pma)) c = X;
                     const int c1 = c ^ b1, dd = c ^ b2, d1 = dd ^ b1;
-                    emitf(o, "      { const cd a = x%d_%d, b = x%d_%d, e = x%d_%d, g = x%d_%d;\n", m, c, m, c1, m, dd, m, d1);
-                    emitf(o, "        x%d_%d = cadd(cmul(f0, cadd(a, b)), cmul(F%d, psum<%d, %d>(e, g)));\n", m, c, m, u(dd), u(d1));
-                    emitf(o, "        x%d_%d = cadd(cmul(f0, cadd(e, g)), cmul(F%d, psum<%d, %d>(a, b))); }\n", m, dd, m, u(c), u(c1));
+                    if (fma2)
+                        add(c, "{ const cd a = x%d_%d, b = x%d_%d, e = x%d_%d, g = x%d_%d; "
+                               "x%d_%d = cfma2(f0, cadd(a, b), F%d, psum<%d, %d>(e, g)); "
+                               "x%d_%d = cfma2(f0, cadd(e, g), F%d, psum<%d, %d>(a, b)); }",
+                            m, c, m, c1, m, dd, m, d1, m, c, m, u(dd), u(d1), m, dd, m, u(c), u(c1));
+                    else
+                        add(c, "{ const cd a = x%d_%d, b = x%d_%d, e = x%d_%d, g = x%d_%d; "
+                               "x%d_%d = cadd(cmul(f0, cadd(a, b)), cmul(F%d, psum<%d, %d>(e, g))); "
+                               "x%d_%d = cadd(cmul(f0, cadd(e, g)), cmul(F%d, psum<%d, %d>(a, b))); }",
+                            m, c, m, c1, m, dd, m, d1, m, c, m, u(dd), u(d1), m, dd, m, u(c), u(c1));
                 }
             } else {
                 const uint32_t PI = q.pat;
@@ -184,33 +216,68 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
                     const int A = par32(E & q.pm) ? EP : E, B = A ^ PI;
                     switch (q.kind) {
                         case 0:
-                            emitf(o, "      { const cd a = x%d_%d; x%d_%d = cmul(f0, a); x%d_%d = sgn<%d>(cmul(F%d, a)); }\n", m, A, m,
-                                  A, m, B, u(A), m);
+                            add(A, "{ const cd a = x%d_%d; x%d_%d = cmul(f0, a); x%d_%d = sgn<%d>(cmul(F%d, a)); }", m, A, m, A, m, B,
+                                u(A), m);
                             break;
                         case 1:
-                            emitf(o, "      { const cd a = x%d_%d, b = x%d_%d;\n", m, A, m, B);
-                            emitf(o, "        x%d_%d = cadd(cmul(f0, a), sgn<%d>(cmul(F%d, b)));\n", m, A, u(B), m);
-                            emitf(o, "        x%d_%d = cadd(cmul(f0, b), sgn<%d>(cmul(F%d, a))); }\n", m, B, u(A), m);
+                            if (fma2)
+                                add(A, "{ const cd a = x%d_%d, b = x%d_%d; x%d_%d = cfma2(f0, a, sgn<%d>(F%d), b); "
+                                       "x%d_%d = cfma2(f0, b, sgn<%d>(F%d), a); }",
+                                    m, A, m, B, m, A, u(B), m, m, B, u(A), m);
+                            else
+                                add(A, "{ const cd a = x%d_%d, b = x%d_%d; x%d_%d = cadd(cmul(f0, a), sgn<%d>(cmul(F%d, b))); "
+                                       "x%d_%d = cadd(cmul(f0, b), sgn<%d>(cmul(F%d, a))); }",
+                                    m, A, m, B, m, A, u(B), m, m, B, u(A), m);
                             break;
                         case 3:
-                            emitf(o, "      { const cd a = x%d_%d, b = x%d_%d;\n", m, A, m, B);
-                            emitf(o, "        x%d_%d = cmul(f0, cadd(a, b)); x%d_%d = cmul(F%d, psum<%d, %d>(a, b)); }\n", m, A, m,
-                                  B, m, u(A), u(B));
+                            add(A, "{ const cd a = x%d_%d, b = x%d_%d; x%d_%d = cmul(f0, cadd(a, b)); "
+                                   "x%d_%d = cmul(F%d, psum<%d, %d>(a, b)); }",
+                                m, A, m, B, m, A, m, B, m, u(A), u(B));
                             break;
                         case 5:
-                            emitf(o, "      { const cd a = x%d_%d, b = x%d_%d;\n", m, A, m, B);
-                            emitf(o, "        x%d_%d = q.y ? cmul(F%d, psum<%d, %d>(a, b)) : cmul(f0, cadd(a, b)); }\n", m, A, m, u(A),
-                                  u(B));
+                            add(A, "{ const cd a = x%d_%d, b = x%d_%d; x%d_%d = q.y ? cmul(F%d, psum<%d, %d>(a, b)) : cmul(f0, cadd(a, b)); }",
+                                m, A, m, B, m, A, m, u(A), u(B));
                             break;
                         default:  // 7
-                            emitf(o, "      x%d_%d = cadd(cmul(W%d_%d, x%d_%d), cmul(W%d_%d, x%d_%d));\n", m, A, m, u(A), m, A, m,
-                                  u(B), m, B);
+                            if (fma2)
+                                add(A, "x%d_%d = cfma2(W%d_%d, x%d_%d, W%d_%d, x%d_%d);", m, A, m, u(A), m, A, m, u(B), m, B);
+                            else
+                                add(A, "x%d_%d = cadd(cmul(W%d_%d, x%d_%d), cmul(W%d_%d, x%d_%d));", m, A, m, u(A), m, A, m, u(B),
+                                    m, B);
                             break;
                     }
                 }
             }
-            if (opt.exact)
-                for (int E = 0; E < NE; ++E) emitf(o, "      x%d_%d = RS(x%d_%d);\n", m, E, m, E);
+            // Liveness: element E of this orbit is live before the step iff (t ^ TE[E]) & dead_t == 0.
+            // The pass pivots of t are zero, so keys with a pass-pivot bit are dead for every orbit.
+            uint32_t pivm = 0;
+            for (int i = 0; i < J; ++i) pivm |= 1u << ps.piv[i];
+            const uint32_t DT = q.dead_t;
+            std::map<uint32_t, std::vector<std::string>> byKey;
+            for (auto &op : ops) {
+                const uint32_t key = TE[op.first] & DT;
+                if (key & pivm) continue;  // statically dead
+                std::string code = op.second;
+                if (opt.exact) {
+                    // round what the update wrote (the outputs of the op) to storage precision
+                    for (int E = 0; E < NE; ++E) {
+                        char pat[32];
+                        snprintf(pat, sizeof(pat), "x%d_%d = ", m, E);
+                        if (code.find(pat) != std::string::npos) {
+                            snprintf(buf, sizeof(buf), " x%d_%d = RS(x%d_%d);", m, E, m, E);
+                            code += buf;
+                        }
+                    }
+                }
+                byKey[key].push_back(code);
+            }
+            const bool runtime = (DT & ~pivm) != 0;
+            if (runtime) emitf(o, "      const uint32_t ct%d = t%d & %uu;\n", m, m, DT);
+            for (auto &kv : byKey) {
+                if (runtime) emitf(o, "      if (ct%d == %uu) {\n", m, kv.first);
+                for (auto &c : kv.second) emitf(o, "      %s\n", c.c_str());
+                if (runtime) emitf(o, "      }\n");
+            }
         }
         emitf(o, "    }\n");
     }
