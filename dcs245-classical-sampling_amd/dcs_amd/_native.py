@@ -18,7 +18,8 @@ MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED, MODE_COMPILED = 0, 1, 2, 3
 
 EXPORTED = (
     'dcs_last_error', 'dcs_get_device_count', 'dcs_get_device_info', 'dcs_context_create',
-    'dcs_context_destroy', 'dcs_synchronize', 'dcs_plan_create', 'dcs_plan_destroy', 'dcs_plan_compile',
+    'dcs_context_destroy', 'dcs_synchronize', 'dcs_plan_create', 'dcs_gather_plan_create', 'dcs_plan_destroy',
+    'dcs_plan_compile',
     'dcs_plan_units',
     'dcs_plan_schedule', 'dcs_contract', 'dcs_block_zero', 'dcs_block_accumulate', 'dcs_block_transform',
     'dcs_block_read', 'dcs_block_device_pointer', 'dcs_draw', 'dcs_frontier_read', 'dcs_plan_profile',
@@ -41,6 +42,16 @@ class PlanDesc(C.Structure):
                 ('open_unitaries', C.POINTER(C.c_double)), ('blocks', C.c_int32),
                 ('fuse_max_dim', C.c_int32), ('resident_bits', C.c_int32), ('tile_log2', C.c_int32),
                 ('exact_rounding', C.c_int32), ('pass_max_dim', C.c_int32), ('lds_storage', C.c_int32)]
+
+
+class GatherDesc(C.Structure):
+    P = C.POINTER
+    _fields_ = [('storage', C.c_int32), ('rmax', C.c_int32), ('nsteps', C.c_int32),
+                ('r_in', P(C.c_int32)), ('r_out', P(C.c_int32)), ('is_free', P(C.c_int32)), ('nbytes', P(C.c_int32)),
+                ('ncombos', P(C.c_int32)), ('tab_off', P(C.c_int64)), ('comb_off', P(C.c_int64)),
+                ('tables', P(C.c_uint64)), ('combos', P(C.c_uint64)), ('beta', P(C.c_uint64)), ('k', C.c_int32),
+                ('terminal_rank', C.c_int32), ('syndrome_map', P(C.c_uint64)), ('phase', P(C.c_double)),
+                ('open_unitaries', P(C.c_double)), ('blocks', C.c_int32)]
 
 
 class DrawResult(C.Structure):
@@ -69,6 +80,7 @@ def load():
         'dcs_context_destroy': (None, [vp]),
         'dcs_synchronize': (C.c_int, [vp]),
         'dcs_plan_create': (C.c_int, [vp, P(PlanDesc), P(vp)]),
+        'dcs_gather_plan_create': (C.c_int, [vp, P(GatherDesc), P(vp)]),
         'dcs_plan_destroy': (None, [vp]),
         'dcs_plan_compile': (C.c_int, [vp, P(dbl)]),
         'dcs_plan_units': (C.c_int, [vp, C.c_int, P(C.c_int)]),
@@ -196,6 +208,48 @@ class Plan:
         self.nsteps, self.k, self.lanes, self.blocks = n, int(k), int(lanes), int(blocks)
         self.storage = int(storage)
         self.terminal_rank = bin(int(terminal_mask)).count('1')
+
+    @classmethod
+    def gather(cls, ctx, steps, rmax, storage, k, terminal_rank, syndrome_map, phase, open_unitaries, blocks=1):
+        """Plan of the canonical gather executor.  steps: make_plan steps (r_in, r_out, free, beta,
+        Lcols, kernel); their byte tables and kernel combinations are built here."""
+        from .scan_plan import byte_tables, kernel_combos
+        self = cls.__new__(cls)
+        self.ctx, self.lib = ctx, ctx.lib
+        m = len(steps)
+        r_in = np.array([s.r_in for s in steps], np.int32)
+        r_out = np.array([s.r_out for s in steps], np.int32)
+        free = np.array([1 if s.free else 0 for s in steps], np.int32)
+        beta = np.array([s.beta for s in steps], np.uint64)
+        tabs = [byte_tables(s.Lcols, s.r_out) for s in steps]
+        combs = [np.asarray(kernel_combos(s.kernel), np.uint64) for s in steps]
+        nbytes = np.array([t.shape[0] for t in tabs], np.int32)
+        ncomb = np.array([len(c) for c in combs], np.int32)
+        tab_off = np.concatenate([[0], np.cumsum([t.size for t in tabs])[:-1]]).astype(np.int64)
+        comb_off = np.concatenate([[0], np.cumsum(ncomb)[:-1]]).astype(np.int64)
+        tables = np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in tabs]), np.uint64)
+        combos = np.ascontiguousarray(np.concatenate(combs), np.uint64)
+        smap = np.ascontiguousarray(syndrome_map, dtype=np.uint64)
+        ph = np.ascontiguousarray(phase, dtype=np.float64)
+        U = np.ascontiguousarray(np.asarray(open_unitaries, dtype=np.complex128).reshape(-1, 4).view(np.float64))
+        keep = (r_in, r_out, free, beta, nbytes, ncomb, tab_off, comb_off, tables, combos, smap, ph, U)
+        d = GatherDesc()
+        d.storage, d.rmax, d.nsteps = int(storage), int(rmax), m
+        d.r_in, d.r_out, d.is_free = ptr(r_in, C.c_int32), ptr(r_out, C.c_int32), ptr(free, C.c_int32)
+        d.nbytes, d.ncombos = ptr(nbytes, C.c_int32), ptr(ncomb, C.c_int32)
+        d.tab_off, d.comb_off = ptr(tab_off, C.c_int64), ptr(comb_off, C.c_int64)
+        d.tables, d.combos, d.beta = ptr(tables, C.c_uint64), ptr(combos, C.c_uint64), ptr(beta, C.c_uint64)
+        d.k, d.terminal_rank = int(k), int(terminal_rank)
+        d.syndrome_map, d.phase, d.open_unitaries = ptr(smap, C.c_uint64), ptr(ph, C.c_double), ptr(U, C.c_double)
+        d.blocks = int(blocks)
+        h = C.c_void_p()
+        check(self.lib.dcs_gather_plan_create(ctx.handle, C.byref(d), C.byref(h)))
+        del keep
+        self.handle = h
+        self.nsteps, self.k, self.lanes, self.blocks = m, int(k), 1, int(blocks)
+        self.storage = int(storage)
+        self.terminal_rank = int(terminal_rank)
+        return self
 
     def compile(self):
         """Generate and compile the plan-specialised group kernels; returns the compile seconds

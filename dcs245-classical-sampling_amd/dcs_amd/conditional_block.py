@@ -7,6 +7,10 @@ sampling/conditional_block_inplace_gpu.py:17-38, kernel_opt/inplace_gpu.py:219-2
     engine.alloc(dtype=np.complex64)            # or np.complex128 for audit blocks
     vector, meta = engine.block(x_bits)         # vector: DeviceBlock (np.asarray(vector) copies it)
 
+ConditionalBlockGPU is the canonical engine of the reference (two tables, the gather executor of
+dcs_rankwidth/gpu_kernel.py); InplaceConditionalBlockGPU replaces its prefix with the one-buffer
+preserved-coordinate executor, as the reference's subclass does.
+
 The host keeps the reference's bookkeeping (local vectors, slice coefficients, offsets, the log-scale
 sum); every pass over a frontier, the slice scatter, the Walsh-Hadamard / phase / Clifford tail and
 the CDF draw run in libdcs_amd.so kernels.  There is no CPU fallback: alloc() raises when the library
@@ -71,8 +75,9 @@ class DeviceBlock:
         return self._engine._plan.draw(self._index, uniform)
 
 
-class InplaceConditionalBlockGPU:
-    """Reference-compatible engine (conditional_block_inplace_gpu.py:194-215)."""
+class ConditionalBlockGPU:
+    """Canonical engine (sampling/conditional_block_gpu.py:24-177): the gather executor over two
+    tables of 2^rmax entries, one launch per step, the reference's scale chain."""
 
     def __init__(self, graph, unitaries, n, order, parity_operations, open_vertices):
         self.bp = bp = BlockPlan(graph, unitaries, n, order, parity_operations, open_vertices)
@@ -89,6 +94,109 @@ class InplaceConditionalBlockGPU:
         self.k = bp.k
         self.profile = False
         self._plan = None
+
+    def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, schedule='steps', blocks=1):
+        """Allocate the two gather tables and upload the plan (dcs_rankwidth/gpu_kernel.py:106-121,
+        conditional_block_gpu.py:71-90).
+
+        dtype      np.complex64 or np.complex128 storage (FP64 arithmetic in both)
+        schedule   'steps' (scale chain on the device, no host synchronisation per step) or 'steps_host'
+                   (the host reads every maximum, as the reference does)
+        The fp64_kernel_path argument is accepted for interface compatibility and ignored."""
+        dt = np.dtype(dtype)
+        if dt not in (np.dtype(np.complex64), np.dtype(np.complex128)):
+            raise ValueError(dtype)
+        if schedule not in ('steps', 'steps_host'):
+            raise ValueError(f'the gather executor runs schedule steps or steps_host, not {schedule!r}')
+        bp = self.bp
+        self.use_fp64 = dt == np.dtype(np.complex128)
+        self.mode = SCHEDULES[schedule]
+        self.lanes = 1
+        self.terminal_basis = bp.gather_terminal_basis
+        self.terminal_rank = len(self.terminal_basis)
+        if self.terminal_rank > bp.k:
+            raise ValueError(f'terminal rank {self.terminal_rank} exceeds the {bp.k} open vertices')
+        self.ctx = _native.Context(device)
+        storage = _native.STORAGE_C128 if self.use_fp64 else _native.STORAGE_C64
+        self._plan = _native.Plan.gather(self.ctx, bp.steps, bp.rmax, storage, bp.k, self.terminal_rank,
+                                         bp.syndrome_map(self.terminal_basis), bp.phase(), bp.open_unitaries(),
+                                         blocks=blocks)
+        self.compile_seconds = 0.0
+        self.buf = True
+
+    # ---- reference mixin hooks (sampling/conditional_block_gpu.py:92-126) -----------------------
+    def _prefix(self, f, yfix):
+        """(env or None, logscale, offset >> stop) of one slice; env is the host copy of the final
+        table in storage precision."""
+        coeffs, ov, ys, offset = self.bp.step_parameters(f, yfix)
+        m2 = self._plan.contract(self.mode, [0], coeffs, ov, ys)[0]
+        ls, alive = logscale_from_maxima(m2)
+        if not alive:
+            return None, ls, 0
+        return self._plan.frontier_terminal(0), ls, offset >> self.stop
+
+    def _sweep(self, f, yfix, renorm_every=None):
+        env, ls, _ = self._prefix(f, yfix)
+        return (0j if env is None else complex(env[0])), ls
+
+    # ---- the block ---------------------------------------------------------------------------
+    def _slice_jobs(self, x_bits, w, part, parts):
+        f_full = self.bp.local_vectors(x_bits, w=w)
+        jobs, count = [], 0
+        for count, coef, f, yfix, freeflip in self.bp.slice_assignments(f_full, part, parts):
+            if f is None:
+                continue
+            coeffs, ov, ys, offset = self.bp.step_parameters(f, yfix)
+            jobs.append((coef, coeffs, ov, ys, offset, freeflip))
+        return jobs, count
+
+    def block(self, x_bits, w=None, part=0, parts=1, block_index=0):
+        """Scaled conditional amplitudes sqrt(D) psi[prefix, :] (conditional_block_gpu.py:128-177).
+
+        Slices are contracted `lanes` at a time; each one's environment is scattered into the
+        complex128 accumulator with its coefficient exp(ls - (N_full - n + k) ln2 / 2), then the
+        tail is applied once."""
+        if self._plan is None:
+            raise RuntimeError('call alloc() first')
+        start = time.perf_counter()
+        plan = self._plan
+        jobs, count = self._slice_jobs(x_bits, w, part, parts)
+        plan.block_zero(block_index)
+        for i in range(0, len(jobs), self.lanes):
+            batch = jobs[i:i + self.lanes]
+            lanes = np.arange(len(batch), dtype=np.int32)
+            m2 = plan.contract(self.mode, lanes, np.stack([j[1] for j in batch]), np.stack([j[2] for j in batch]),
+                               np.stack([j[3] for j in batch]))
+            for lane, (coef, _, _, _, offset, freeflip) in enumerate(batch):
+                ls, alive = logscale_from_maxima(m2[lane])
+                if not alive:
+                    continue
+                c = coef * self.bp.slice_scale(ls)
+                plan.block_accumulate(block_index, lane, (offset >> self.stop) ^ freeflip, c)
+        plan.block_transform(block_index)
+        self.ctx.synchronize()
+        vec = DeviceBlock(self, block_index)
+        return vec, {'seconds': time.perf_counter() - start, 'part': part, 'parts': parts,
+                     'slices_evaluated': count,
+                     'prefix_dtype': 'complex128' if self.use_fp64 else 'complex64',
+                     'rmax': self.rmax, 'terminal_rank': self.terminal_rank,
+                     'work_per_slice': self.work, 'open_vertices': self.open_full,
+                     'vector_scale': 'sqrt(2^(n-k)); sum all complex parts before probabilities'}
+
+    def schedule(self):
+        return self._plan.schedule(self.mode)
+
+    def close(self):
+        if self._plan is not None:
+            self._plan.close()
+            self._plan = None
+        if getattr(self, 'ctx', None) is not None:
+            self.ctx.close()
+            self.ctx = None
+
+
+class InplaceConditionalBlockGPU(ConditionalBlockGPU):
+    """Reference-compatible in-place engine (conditional_block_inplace_gpu.py:17-38, 194-215)."""
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='fused',
               fuse_max_dim=6, resident_bits=12, tile_log2=11, blocks=1, exact_rounding=False,
@@ -148,61 +256,3 @@ class InplaceConditionalBlockGPU:
     def _sweep(self, f, yfix, renorm_every=None):
         env, ls, _ = self.prefix_inplace(f, yfix)
         return (0j if env is None else complex(env[0])), ls
-
-    # ---- the block ---------------------------------------------------------------------------
-    def _slice_jobs(self, x_bits, w, part, parts):
-        f_full = self.bp.local_vectors(x_bits, w=w)
-        jobs, count = [], 0
-        for count, coef, f, yfix, freeflip in self.bp.slice_assignments(f_full, part, parts):
-            if f is None:
-                continue
-            coeffs, ov, ys, offset = self.bp.step_parameters(f, yfix)
-            jobs.append((coef, coeffs, ov, ys, offset, freeflip))
-        return jobs, count
-
-    def block(self, x_bits, w=None, part=0, parts=1, block_index=0):
-        """Scaled conditional amplitudes sqrt(D) psi[prefix, :] (conditional_block_gpu.py:128-177).
-
-        Slices are contracted `lanes` at a time; each one's environment is scattered into the
-        complex128 accumulator with its coefficient exp(ls - (N_full - n + k) ln2 / 2), then the
-        tail is applied once."""
-        if self._plan is None:
-            raise RuntimeError('call alloc() first')
-        start = time.perf_counter()
-        plan = self._plan
-        jobs, count = self._slice_jobs(x_bits, w, part, parts)
-        plan.block_zero(block_index)
-        for i in range(0, len(jobs), self.lanes):
-            batch = jobs[i:i + self.lanes]
-            lanes = np.arange(len(batch), dtype=np.int32)
-            m2 = plan.contract(self.mode, lanes, np.stack([j[1] for j in batch]), np.stack([j[2] for j in batch]),
-                               np.stack([j[3] for j in batch]))
-            for lane, (coef, _, _, _, offset, freeflip) in enumerate(batch):
-                ls, alive = logscale_from_maxima(m2[lane])
-                if not alive:
-                    continue
-                c = coef * self.bp.slice_scale(ls)
-                plan.block_accumulate(block_index, lane, (offset >> self.stop) ^ freeflip, c)
-        plan.block_transform(block_index)
-        self.ctx.synchronize()
-        vec = DeviceBlock(self, block_index)
-        return vec, {'seconds': time.perf_counter() - start, 'part': part, 'parts': parts,
-                     'slices_evaluated': count,
-                     'prefix_dtype': 'complex128' if self.use_fp64 else 'complex64',
-                     'rmax': self.rmax, 'terminal_rank': self.terminal_rank,
-                     'work_per_slice': self.work, 'open_vertices': self.open_full,
-                     'vector_scale': 'sqrt(2^(n-k)); sum all complex parts before probabilities'}
-
-    def schedule(self):
-        return self._plan.schedule(self.mode)
-
-    def close(self):
-        if self._plan is not None:
-            self._plan.close()
-            self._plan = None
-        if getattr(self, 'ctx', None) is not None:
-            self.ctx.close()
-            self.ctx = None
-
-
-ConditionalBlockGPU = InplaceConditionalBlockGPU

@@ -15,10 +15,11 @@
 //   k_scatter, k_wht_pass, k_phase, k_u_pass
 //               coherent slice sum and the Walsh-Hadamard / phase / local-Clifford tail
 //               (sampling/conditional_block_gpu.py:155-171).
-//   k_cumsum, k_search
+//   k_weights, k_chain, k_search
 //               |a|^2, the sequential float64 cumsum, normalisation and searchsorted(side='right')
-//               of draw_checked (sampling/sampling_protocol.py:56-65); the single-wave sequential
-//               scan reproduces numpy's accumulation order bit for bit.
+//               of draw_checked (sampling/sampling_protocol.py:56-65); the running sum is one
+//               dependent chain in numpy's accumulation order, so every cumulative value is
+//               bit-identical to cumsum's.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
@@ -395,6 +396,46 @@ __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 la
 }
 
 // ------------------------------------------------------------------------------------------------
+// k_gather: one step of the canonical gather executor (dcs_rankwidth/gpu_kernel.py:58-101) with the
+// fused power-of-two scale and max |out|^2.  The step's byte tables (<= 5 x 256 u64) and kernel
+// combinations are staged in LDS.  Sum over combinations in their order, products e * w with the
+// library's complex-product convention.
+struct GatherArgs {
+    u64 n_out, mask, beta;
+    const u64 *T, *combos;
+    int32_t r_in, nbytes, ncombos, is_free;
+};
+
+template <class S>
+__global__ void __launch_bounds__(kThreads) k_gather(const S *__restrict__ Ein, S *__restrict__ Eout, GatherArgs a,
+                                                     const Dyn *__restrict__ dyn, int step, u64 *__restrict__ dmax,
+                                                     int unit) {
+    __shared__ u64 tab[5 * 256];
+    __shared__ u64 cmb[8];
+    for (int i = threadIdx.x; i < a.nbytes * 256; i += blockDim.x) tab[i] = a.T[i];
+    if ((int)threadIdx.x < a.ncombos) cmb[threadIdx.x] = a.combos[threadIdx.x];
+    __syncthreads();
+    const Dyn dd = dyn[step];
+    const cd f0{dd.f0r, dd.f0i}, f1{dd.f1r, dd.f1i};
+    const double scale = scale_from_m2(unit ? bits2d(dmax[unit - 1]) : 1.0);
+    double lm = 0.0;
+    for (u64 cp = (u64)blockIdx.x * blockDim.x + threadIdx.x; cp < a.n_out; cp += (u64)blockDim.x * gridDim.x) {
+        u64 p0 = 0;
+        for (int b = 0; b < a.nbytes; ++b) p0 ^= tab[b * 256 + ((cp >> (8 * b)) & 255)];
+        cd acc{0.0, 0.0};
+        for (int k = 0; k < a.ncombos; ++k) {
+            const u64 p = p0 ^ cmb[k];
+            const u64 c = p & a.mask;
+            const int par = (__popcll(c & a.beta) & 1) ^ dd.ov;
+            const int yv = a.is_free ? (int)((p >> a.r_in) & 1) : dd.y;
+            acc = cadd(acc, cmul(Store<S>::load(Ein[c]), cneg_if(yv ? f1 : f0, yv & par)));
+        }
+        put_scaled(Eout, cp, acc, scale, lm);
+    }
+    commit_max(lm, &dmax[unit]);
+}
+
+// ------------------------------------------------------------------------------------------------
 __global__ void k_fill_u64(u64 *p, u64 n, u64 v) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x) p[i] = v;
 }
@@ -453,57 +494,121 @@ __global__ void k_u_pass(double2 *v, u64 half, int q, cd u00, cd u01, cd u10, cd
     }
 }
 
-// One wavefront: weights w = re*re + im*im and their strictly sequential float64 running sum.
-__global__ void __launch_bounds__(64) k_cumsum(const double2 *__restrict__ v, u64 n, double *__restrict__ cum) {
-    const int lane = threadIdx.x;
-    double acc = 0.0;
-    for (u64 base = 0; base < n; base += 64) {
-        const u64 i = base + lane;
-        double w = 0.0;
-        if (i < n) {
-            const double2 a = v[i];
-            w = a.x * a.x + a.y * a.y;
-        }
-        double mine = 0.0;
-#pragma unroll
-        for (int t = 0; t < 64; ++t) {
-            acc = acc + __shfl(w, t);
-            if (lane == t) mine = acc;
-        }
-        if (i < n) cum[i] = mine;
+// Weights w = re*re + im*im of the block (numpy's real**2 + imag**2, unfused).
+__global__ void k_weights(const double2 *__restrict__ v, u64 n, double *__restrict__ w) {
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x) {
+        const double2 a = v[i];
+        w[i] = a.x * a.x + a.y * a.y;
     }
 }
 
-__global__ void k_search(const double2 *__restrict__ v, const double *__restrict__ cum, u64 n, double u,
+// The strictly sequential float64 running sum of numpy's cumsum, as one dependent chain of adds in a
+// single wavefront.  Chunks of 512 weights are staged in LDS (the next chunk is loaded into registers
+// while the chain runs over the current one) and read back with wave-uniform addresses.  Only every
+// 64th running sum is kept,
+//   ck[j] = w[0] + ... + w[64 j + 63]  (j < n / 64),  ck[n / 64] = the total;
+// k_search recomputes the 64 sums of one segment from ck[j - 1] in the same order, so every
+// cumulative value it uses is bit-identical to cumsum's.
+__global__ void __launch_bounds__(64) k_chain(const double *__restrict__ w, u64 n, double *__restrict__ ck) {
+    __shared__ double seg[512];
+    const int lane = threadIdx.x;
+    const u64 nchunk = (n + 511) >> 9;
+    double r[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const u64 i = (u64)k * 64 + lane;
+        r[k] = i < n ? w[i] : 0.0;
+    }
+    double acc = 0.0;
+    for (u64 c = 0; c < nchunk; ++c) {
+        const u64 base = c << 9;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) seg[k * 64 + lane] = r[k];
+        __syncthreads();
+        if (c + 1 < nchunk) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const u64 i = base + 512 + (u64)k * 64 + lane;
+                r[k] = i < n ? w[i] : 0.0;
+            }
+        }
+        if (base + 512 <= n) {
+            // register double buffer of 32 operands: the LDS reads of the next 32 are issued before
+            // the 32 dependent adds of the current ones (the scheduling barrier keeps that order)
+            double cur[32], nxt[32];
+#pragma unroll
+            for (int t = 0; t < 32; ++t) cur[t] = seg[t];
+#pragma unroll
+            for (int h = 0; h < 16; ++h) {
+                if (h + 1 < 16) {
+#pragma unroll
+                    for (int t = 0; t < 32; ++t) nxt[t] = seg[(h + 1) * 32 + t];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int t = 0; t < 32; ++t) acc = acc + cur[t];
+                if (h & 1) ck[(base >> 6) + (h >> 1)] = acc;
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int t = 0; t < 32; ++t) cur[t] = nxt[t];
+            }
+        } else {
+            const int cnt = (int)(n - base);
+            for (int t = 0; t < cnt; ++t) {
+                acc = acc + seg[t];
+                if (((t + 1) & 63) == 0) ck[(base + t) >> 6] = acc;
+            }
+        }
+        __syncthreads();
+    }
+    ck[n >> 6] = acc;
+}
+
+// searchsorted(cdf, u, side='right') on cdf = cumsum(w) / total with cdf[-1] forced to 1
+// (sampling/sampling_protocol.py:56-65): binary search over the segment ends, then the segment's
+// running sums in order.
+__global__ void k_search(const double *__restrict__ w, const double *__restrict__ ck, u64 n, double u,
                          DrawOut *out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     DrawOut r;
     memset(&r, 0, sizeof(r));
-    const double total = cum[n - 1];
+    const u64 nseg = n >> 6;
+    const double total = ck[nseg];
     r.total = total;
     if (!(total > 0.0) || !isfinite(total)) {
         r.status = isfinite(total) ? DCS_ERR_ZERO : DCS_ERR_NONFINITE;
         *out = r;
         return;
     }
-    // normalised cdf: cum[i] / total, last entry forced to 1
-    u64 lo = 0, hi = n;
+    auto cdf_end = [&](u64 j) { return ((j << 6) + 63 == n - 1) ? 1.0 : ck[j] / total; };
+    u64 lo = 0, hi = nseg;
     while (lo < hi) {
         const u64 mid = (lo + hi) >> 1;
-        const double c = (mid == n - 1) ? 1.0 : cum[mid] / total;
-        if (c <= u) lo = mid + 1;
+        if (cdf_end(mid) <= u) lo = mid + 1;
         else hi = mid;
     }
-    r.index = (int64_t)lo;
-    if (lo >= n) {
+    const u64 s0 = lo << 6, s1 = (s0 + 64 < n) ? s0 + 64 : n;
+    double acc = lo ? ck[lo - 1] : 0.0;
+    double lower = lo ? acc / total : 0.0;
+    u64 idx = n;
+    for (u64 i = s0; i < s1; ++i) {
+        acc = acc + w[i];
+        const double c = (i == n - 1) ? 1.0 : acc / total;
+        if (c > u) {
+            idx = i;
+            r.upper = c;
+            break;
+        }
+        lower = c;
+    }
+    r.index = (int64_t)idx;
+    if (idx >= n) {
         r.status = DCS_ERR_ZERO;
         *out = r;
         return;
     }
-    const double2 a = v[lo];
-    r.weight = a.x * a.x + a.y * a.y;
-    r.lower = lo ? ((lo - 1 == n - 1) ? 1.0 : cum[lo - 1] / total) : 0.0;
-    r.upper = (lo == n - 1) ? 1.0 : cum[lo] / total;
+    r.weight = w[idx];
+    r.lower = idx ? lower : 0.0;
     r.margin = fmin(u - r.lower, r.upper - u);
     r.status = (r.weight > 0.0) ? DCS_OK : DCS_ERR_ZERO;
     *out = r;
@@ -568,6 +673,11 @@ struct dcs_plan {
     std::vector<PassDev> h_passes;
     std::vector<PStepDev> h_psteps;
     std::vector<u64> h_addrv;
+    // canonical gather executor (dcs_gather_plan_create): frontier holds the current table
+    int executor = 0;  // 0 in-place, 1 gather
+    std::vector<GatherArgs> gsteps;
+    void *frontier2 = nullptr;
+    u64 *d_gtables = nullptr, *d_gcombos = nullptr;
     // DCS_MODE_COMPILED
     bool compiled = false;
     std::vector<hipFunction_t> group_fn;
@@ -1026,6 +1136,7 @@ ModuleCache &module_cache() {
 
 int compile_groups(dcs_plan *p, double *seconds) {
     if (seconds) *seconds = 0.0;
+    if (p->executor != 0) return fail(DCS_ERR_INVALID, "a gather plan has no coset groups to compile");
     if (p->compiled) return DCS_OK;
     CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0};
     std::string src = codegen::prelude(opt, kKernelsHpp);
@@ -1095,6 +1206,34 @@ int launch_compiled(dcs_plan *p, int nl, const Seg &sg, int nunits) {
     void *args[] = {&E, &ls, &lanes, &dyn, &nsteps, &dmax, &nu, &unit};
     PROF(2, HIPCHK(hipModuleLaunchKernel(p->group_fn[sg.group], p->group_grid[sg.group], (unsigned)nl, 1, kThreads, 1, 1, 0,
                                          p->ctx->stream, args, nullptr)));
+    return DCS_OK;
+}
+
+template <class S>
+int run_gather(dcs_plan *p, int mode) {
+    const int n = p->nsteps;
+    k_fill_u64<<<grid_for(n, p->ctx->cus, 4), kThreads, 0, p->ctx->stream>>>(p->d_dmax, n, 0ull);
+    HIPCHK(hipGetLastError());
+    k_init_lanes<S><<<1, 64, 0, p->ctx->stream>>>((S *)p->frontier, p->lane_stride, p->d_lanes, 1);
+    HIPCHK(hipGetLastError());
+    for (int s = 0; s < n; ++s) {
+        const GatherArgs &a = p->gsteps[s];
+        const unsigned g = grid_for(a.n_out, p->ctx->cus, 32);
+        PROF(0, (k_gather<S><<<g, kThreads, 0, p->ctx->stream>>>((const S *)p->frontier, (S *)p->frontier2, a, p->d_dyn, s,
+                                                                 p->d_dmax, s)));
+        std::swap(p->frontier, p->frontier2);
+        if (mode == DCS_MODE_STEPS_HOST) {
+            // the reference reads the maximum back after every step (conditional_block_gpu.py:118-122)
+            HIPCHK(hipMemcpyAsync(p->h_dmax, p->d_dmax + s, sizeof(u64), hipMemcpyDeviceToHost, p->ctx->stream));
+            HIPCHK(hipStreamSynchronize(p->ctx->stream));
+            double m2;
+            memcpy(&m2, p->h_dmax, 8);
+            if (!std::isfinite(m2) || m2 == 0.0) {
+                // leave the table that holds the failing step's output in place, as the reference does
+                break;
+            }
+        }
+    }
     return DCS_OK;
 }
 
@@ -1209,7 +1348,8 @@ void dcs_plan_destroy(dcs_plan *p) {
         (void)hipEventDestroy(pe.b);
     }
     void *dev[] = {p->frontier, p->d_dyn, p->d_lanes, p->d_dmax, p->d_steps, p->d_groups, p->d_passes, p->d_psteps,
-                   p->d_addrv, p->d_smap, p->d_phase, p->d_blocks, p->d_cum, p->d_scratch, p->d_draw};
+                   p->d_addrv, p->d_smap, p->d_phase, p->d_blocks, p->d_cum, p->d_scratch, p->d_draw, p->frontier2,
+                   p->d_gtables, p->d_gcombos};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     void *host[] = {p->h_dyn, p->h_lanes, p->h_dmax, p->h_draw};
@@ -1389,6 +1529,117 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
 }
 
 
+int dcs_gather_plan_create(dcs_context *ctx, const dcs_gather_desc *d, dcs_plan **out) {
+    if (!ctx || !d || !out) return fail(DCS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (d->storage != DCS_STORAGE_C64 && d->storage != DCS_STORAGE_C128)
+        return fail(DCS_ERR_INVALID, "storage must be 0 (complex64) or 1 (complex128)");
+    if (d->rmax < 0 || d->rmax > 40) return fail(DCS_ERR_INVALID, "rmax %d out of range", d->rmax);
+    if (d->nsteps < 1 || d->blocks < 1) return fail(DCS_ERR_INVALID, "nsteps and blocks must be >= 1");
+    if (d->k < 0 || d->k > 30 || d->terminal_rank < 0 || d->terminal_rank > d->k)
+        return fail(DCS_ERR_INVALID, "terminal rank %d outside 0..k (k=%d)", d->terminal_rank, d->k);
+    if (!d->r_in || !d->r_out || !d->is_free || !d->nbytes || !d->ncombos || !d->tab_off || !d->comb_off || !d->tables ||
+        !d->combos || !d->beta || !d->syndrome_map || !d->phase || !d->open_unitaries)
+        return fail(DCS_ERR_INVALID, "null table");
+    int64_t ntab = 0, ncomb = 0;
+    for (int s = 0; s < d->nsteps; ++s) {
+        if (d->r_in[s] < 0 || d->r_in[s] > d->rmax || d->r_out[s] < 0 || d->r_out[s] > d->rmax)
+            return fail(DCS_ERR_INVALID, "step %d: table bits outside 0..rmax", s);
+        if (s > 0 && d->r_in[s] != d->r_out[s - 1]) return fail(DCS_ERR_INVALID, "step %d: tables do not chain", s);
+        if (s == 0 && d->r_in[s] != 0) return fail(DCS_ERR_INVALID, "the first step must read the one-entry table");
+        if (d->nbytes[s] < 0 || d->nbytes[s] > 5 || d->nbytes[s] * 8 < d->r_out[s])
+            return fail(DCS_ERR_INVALID, "step %d: %d byte tables for %d output bits", s, d->nbytes[s], d->r_out[s]);
+        if (d->ncombos[s] < 1 || d->ncombos[s] > 8) return fail(DCS_ERR_INVALID, "step %d: 1..8 kernel combinations", s);
+        if (d->tab_off[s] < 0 || d->comb_off[s] < 0) return fail(DCS_ERR_INVALID, "step %d: negative offset", s);
+        ntab = std::max<int64_t>(ntab, d->tab_off[s] + (int64_t)d->nbytes[s] * 256);
+        ncomb = std::max<int64_t>(ncomb, d->comb_off[s] + d->ncombos[s]);
+    }
+    if (d->r_out[d->nsteps - 1] != d->terminal_rank)
+        return fail(DCS_ERR_INVALID, "the last table has %d bits, terminal rank %d", d->r_out[d->nsteps - 1], d->terminal_rank);
+    HIPCHK(hipSetDevice(ctx->device));
+    dcs_plan *p = new dcs_plan();
+    auto bail = [&](int rc) {
+        dcs_plan_destroy(p);
+        return rc;
+    };
+    p->ctx = ctx;
+    p->executor = 1;
+    p->storage = d->storage;
+    p->maxslots = d->rmax;
+    p->lanes = 1;
+    p->nsteps = d->nsteps;
+    p->k = d->k;
+    p->blocks = d->blocks;
+    p->esz = d->storage == DCS_STORAGE_C64 ? sizeof(float2) : sizeof(double2);
+    p->lane_stride = 1ull << d->rmax;
+    p->terminal_mask = (d->terminal_rank == 0) ? 0ull : ((1ull << d->terminal_rank) - 1);
+    p->unitaries.assign(d->open_unitaries, d->open_unitaries + (size_t)d->k * 8);
+    for (int s = 0; s < d->nsteps; ++s) p->sched_steps.push_back(Seg{0, s, 1, 0, -1, s});
+    p->sched_fused = p->sched_steps;
+    p->units_steps = p->units_fused = d->nsteps;
+    const u64 table_bytes = p->lane_stride * p->esz;
+    const u64 block_len = 1ull << d->k;
+    const u64 aux_bytes = block_len * (8 + 8 + 16 * (u64)d->blocks + 8 + 16) + (64ull << 20) + 8ull * (ntab + ncomb);
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return bail(fail(DCS_ERR_RUNTIME, "hipMemGetInfo failed"));
+    if (2 * table_bytes + aux_bytes > fr)
+        return bail(fail(DCS_ERR_NOMEM, "two tables need %llu bytes (+%llu auxiliary), %llu free",
+                         (unsigned long long)(2 * table_bytes), (unsigned long long)aux_bytes, (unsigned long long)fr));
+#define DALLOC(ptr, bytes)                                                                       \
+    do {                                                                                         \
+        if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess)                                   \
+            return bail(fail(DCS_ERR_NOMEM, "hipMalloc of %llu bytes failed", (unsigned long long)(bytes))); \
+    } while (0)
+#define HALLOC(ptr, bytes)                                                                       \
+    do {                                                                                         \
+        if (hipHostMalloc((void **)&(ptr), (bytes), hipHostMallocDefault) != hipSuccess)         \
+            return bail(fail(DCS_ERR_NOMEM, "hipHostMalloc of %llu bytes failed", (unsigned long long)(bytes))); \
+    } while (0)
+    DALLOC(p->frontier, table_bytes);
+    DALLOC(p->frontier2, table_bytes);
+    DALLOC(p->d_gtables, sizeof(u64) * std::max<int64_t>(1, ntab));
+    DALLOC(p->d_gcombos, sizeof(u64) * std::max<int64_t>(1, ncomb));
+    DALLOC(p->d_dyn, sizeof(Dyn) * (size_t)d->nsteps);
+    DALLOC(p->d_lanes, sizeof(int));
+    DALLOC(p->d_dmax, sizeof(u64) * (size_t)d->nsteps);
+    DALLOC(p->d_smap, sizeof(u64) * block_len);
+    DALLOC(p->d_phase, sizeof(double) * block_len);
+    DALLOC(p->d_blocks, sizeof(double2) * block_len * d->blocks);
+    DALLOC(p->d_cum, sizeof(double) * block_len);
+    DALLOC(p->d_scratch, sizeof(double2) * block_len);
+    DALLOC(p->d_draw, sizeof(DrawOut));
+    HALLOC(p->h_dyn, sizeof(Dyn) * (size_t)d->nsteps);
+    HALLOC(p->h_lanes, sizeof(int));
+    HALLOC(p->h_dmax, sizeof(u64) * (size_t)d->nsteps);
+    HALLOC(p->h_draw, sizeof(DrawOut));
+#undef DALLOC
+#undef HALLOC
+    hipStream_t st = ctx->stream;
+    if (hipMemcpyAsync(p->d_gtables, d->tables, sizeof(u64) * ntab, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(p->d_gcombos, d->combos, sizeof(u64) * ncomb, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(p->d_smap, d->syndrome_map, sizeof(u64) * ((u64)1 << d->terminal_rank), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(p->d_phase, d->phase, sizeof(double) * block_len, hipMemcpyHostToDevice, st) != hipSuccess)
+        return bail(fail(DCS_ERR_RUNTIME, "upload failed"));
+    if (hipStreamSynchronize(st) != hipSuccess) return bail(fail(DCS_ERR_RUNTIME, "plan upload sync failed"));
+    for (int s = 0; s < d->nsteps; ++s) {
+        GatherArgs a;
+        memset(&a, 0, sizeof(a));
+        a.n_out = 1ull << d->r_out[s];
+        a.mask = (d->r_in[s] == 0) ? 0ull : ((1ull << d->r_in[s]) - 1);
+        a.beta = d->beta[s];
+        a.T = p->d_gtables + d->tab_off[s];
+        a.combos = p->d_gcombos + d->comb_off[s];
+        a.r_in = d->r_in[s];
+        a.nbytes = d->nbytes[s];
+        a.ncombos = d->ncombos[s];
+        a.is_free = d->is_free[s] ? 1 : 0;
+        p->gsteps.push_back(a);
+    }
+    *out = p;
+    return DCS_OK;
+}
+
 int dcs_plan_compile(dcs_plan *p, double *seconds) {
     if (!p) return fail(DCS_ERR_INVALID, "null plan");
     return compile_groups(p, seconds);
@@ -1420,6 +1671,8 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
                  const int32_t *y, double *unit_m2) {
     if (!p || !lanes || !coeffs || !ov || !y || !unit_m2) return fail(DCS_ERR_INVALID, "null argument");
     if (mode < 0 || mode > 3) return fail(DCS_ERR_INVALID, "mode %d", mode);
+    if (p->executor == 1 && mode != DCS_MODE_STEPS && mode != DCS_MODE_STEPS_HOST)
+        return fail(DCS_ERR_INVALID, "a gather plan runs DCS_MODE_STEPS or DCS_MODE_STEPS_HOST");
     if (nl < 1 || nl > p->lanes) return fail(DCS_ERR_INVALID, "nlanes %d outside 1..%d", nl, p->lanes);
     for (int i = 0; i < nl; ++i) {
         if (lanes[i] < 0 || lanes[i] >= p->lanes) return fail(DCS_ERR_INVALID, "lane %d out of range", lanes[i]);
@@ -1448,8 +1701,12 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
     hipStream_t st = p->ctx->stream;
     HIPCHK(hipMemcpyAsync(p->d_dyn, p->h_dyn, sizeof(Dyn) * (size_t)nl * n, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(p->d_lanes, p->h_lanes, sizeof(int) * nl, hipMemcpyHostToDevice, st));
-    const int rc = (p->storage == DCS_STORAGE_C64) ? run_contract<float2>(p, mode, nl, p->group_ntiles)
-                                                   : run_contract<double2>(p, mode, nl, p->group_ntiles);
+    int rc;
+    if (p->executor == 1)
+        rc = (p->storage == DCS_STORAGE_C64) ? run_gather<float2>(p, mode) : run_gather<double2>(p, mode);
+    else
+        rc = (p->storage == DCS_STORAGE_C64) ? run_contract<float2>(p, mode, nl, p->group_ntiles)
+                                             : run_contract<double2>(p, mode, nl, p->group_ntiles);
     if (rc) {
         (void)hipStreamSynchronize(st);
         return rc;
@@ -1527,8 +1784,10 @@ int dcs_draw(dcs_plan *p, int block, double u, dcs_draw_result *out) {
     HIPCHK(hipSetDevice(p->ctx->device));
     const u64 n = 1ull << p->k;
     const double2 *v = p->d_blocks + ((u64)block << p->k);
-    PROF(4, (k_cumsum<<<1, 64, 0, p->ctx->stream>>>(v, n, p->d_cum)));
-    PROF(4, (k_search<<<1, 64, 0, p->ctx->stream>>>(v, p->d_cum, n, u, p->d_draw)));
+    double *ck = (double *)p->d_scratch;  // n / 64 + 1 doubles
+    PROF(4, (k_weights<<<grid_for(n, p->ctx->cus, 4), kThreads, 0, p->ctx->stream>>>(v, n, p->d_cum)));
+    PROF(4, (k_chain<<<1, 64, 0, p->ctx->stream>>>(p->d_cum, n, ck)));
+    PROF(4, (k_search<<<1, 64, 0, p->ctx->stream>>>(p->d_cum, ck, n, u, p->d_draw)));
     HIPCHK(hipMemcpyAsync(p->h_draw, p->d_draw, sizeof(DrawOut), hipMemcpyDeviceToHost, p->ctx->stream));
     HIPCHK(hipStreamSynchronize(p->ctx->stream));
     int rc = ev_resolve(p);

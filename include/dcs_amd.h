@@ -21,6 +21,10 @@
  *   dcs_draw                  draw_checked primary CDF (cumsum,       sampling/sampling_protocol.py:56-65,81-82
  *                             searchsorted side='right')
  *   dcs_frontier_read         d_inplace[d_inplace_terminal_indices]   kernel_opt/inplace_gpu.py:237-239,259
+ *   dcs_gather_plan_create    GPUMixin.alloc + ConditionalBlockGPU     dcs_rankwidth/gpu_kernel.py:106-121
+ *                             .alloc (canonical gather executor)       sampling/conditional_block_gpu.py:71-90
+ *   dcs_contract on a gather  ConditionalBlockGPU._prefix (one launch   sampling/conditional_block_gpu.py:92-126
+ *   plan                      of scan_step_scaled per step)            dcs_rankwidth/gpu_kernel.py:58-101
  */
 #ifndef DCS_AMD_H
 #define DCS_AMD_H
@@ -124,6 +128,35 @@ void dcs_plan_destroy(dcs_plan *plan);
  * *seconds (may be NULL) receives the compile time (0 when served from the cache).  dcs_contract in
  * DCS_MODE_COMPILED calls it on first use. */
 int dcs_plan_compile(dcs_plan *plan, double *seconds);
+/* Canonical gather executor: two tables of 2^rmax entries; step s maps the 2^r_in table to the
+ * 2^r_out table, out[c'] = sum over the step's kernel combinations of E[p & (2^r_in - 1)] * (+-f_y),
+ * p = XOR_b tables[b][byte b of c'] ^ combination.  dcs_contract on such a plan accepts
+ * DCS_MODE_STEPS (scale chain on the device) and DCS_MODE_STEPS_HOST, one lane; afterwards the final
+ * table's first 2^terminal_rank entries are the terminal environment (entry i = coordinate i). */
+typedef struct {
+    int32_t storage;            /* DCS_STORAGE_* */
+    int32_t rmax;               /* table address bits, <= 40 */
+    int32_t nsteps;
+    const int32_t *r_in;        /* per step: input / output table bits, free vertex flag */
+    const int32_t *r_out;
+    const int32_t *is_free;
+    const int32_t *nbytes;      /* per step: byte tables (<= 5) and kernel combinations (<= 8) */
+    const int32_t *ncombos;
+    const int64_t *tab_off;     /* per step: offset of its tables / combinations in the arrays below */
+    const int64_t *comb_off;
+    const uint64_t *tables;     /* [nbytes][256] per step, concatenated */
+    const uint64_t *combos;
+    const uint64_t *beta;       /* per step: sign mask over the input coordinates */
+    int32_t k;                  /* log2 of the block length */
+    int32_t terminal_rank;      /* r <= k */
+    const uint64_t *syndrome_map;   /* [2^r] */
+    const double *phase;            /* [2^k] */
+    const double *open_unitaries;   /* [k][8] */
+    int32_t blocks;
+} dcs_gather_desc;
+
+int dcs_gather_plan_create(dcs_context *ctx, const dcs_gather_desc *desc, dcs_plan **out);
+
 /* Number of scale units (maxima) a contraction in `mode` produces per lane. */
 int dcs_plan_units(dcs_plan *plan, int mode, int *nunits);
 /* Schedule of `mode`: per segment (kind 0 step, 1 resident run, 2 coset group), first step, step

@@ -7,6 +7,10 @@
  *                          updates) with the scale chain of the GPU driver
  *                          kernel_opt/inplace_gpu.py:242-258 (step j multiplied by 2^-e,
  *                          e = frexp(sqrt(max |out_{j-1}|^2)).exp; max of the stored values)
+ *   oracle_gather          dcs_rankwidth/scan_plan.py:161-195 and dcs_rankwidth/gpu_kernel.py:58-101
+ *                          (the canonical gather executor: out[c'] = sum over kernel combinations of
+ *                          E[p & mask] * (+-f_y), p = XOR of byte tables ^ combination) with the
+ *                          fused power-of-two scale of sampling/conditional_block_gpu.py:92-126
  *   oracle_accumulate      sampling/conditional_block_gpu.py:155-158 (total[smap ^ x] += env * coef)
  *   oracle_transform       sampling/conditional_block_gpu.py:159-171 (Walsh-Hadamard, phase, U)
  *   oracle_draw            sampling/sampling_protocol.py:56-65,81-82 (weights, sequential cumsum,
@@ -192,6 +196,66 @@ int oracle_contract(int storage, int nsteps, const int32_t *kind, const u64 *K, 
         prev = m;
     }
     return nsteps;
+}
+
+/*
+ * Canonical gather executor over two caller-owned buffers A, B of 2^rmax entries.  Step s reads the
+ * 2^r_in[s] entries of the current table and writes 2^r_out[s] entries of the other one:
+ *   p0 = XOR_b tables[b][byte_b(c')],  for each combination k: p = p0 ^ combos[k], c = p & (2^r_in - 1),
+ *   w = f_{y} with y = bit r_in of p (free vertex) or y[s] (fixed), negated when y and
+ *   parity(c & beta) ^ ov is odd,  out[c'] = round(scale * sum_k E[c] * w)  (sum in combination order).
+ * Tables and combinations are concatenated; tab_off / comb_off index them per step.  Returns the
+ * number of the table holding the result (0: A, 1: B) or -1 - s when step s produced a zero or
+ * nonfinite maximum.
+ */
+int oracle_gather(int storage, int nsteps, const int32_t *r_in, const int32_t *r_out, const int32_t *is_free,
+                  const int32_t *nbytes, const int64_t *tab_off, const u64 *tables, const int32_t *ncombos,
+                  const int64_t *comb_off, const u64 *combos, const u64 *beta, const double *coeffs,
+                  const int32_t *ov, const int32_t *y, void *A, void *B, double *unit_m2) {
+    void *buf[2] = {A, B};
+    int cur = 0;
+    if (storage) { ((double *)A)[0] = 1.0; ((double *)A)[1] = 0.0; }
+    else { ((float *)A)[0] = 1.0f; ((float *)A)[1] = 0.0f; }
+    double prev = 1.0;
+    for (int s = 0; s < nsteps; ++s) {
+        const long long nout = 1ll << r_out[s];
+        const u64 mask = (r_in[s] >= 64) ? ~0ull : ((1ull << r_in[s]) - 1);
+        const u64 *T = tables + tab_off[s];
+        const u64 *cb = combos + comb_off[s];
+        const int nb = nbytes[s], nc = ncombos[s], fr = is_free[s], ri = r_in[s], o = ov[s] & 1, yy = y[s] & 1;
+        const u64 bt = beta[s];
+        const cd f0 = {coeffs[4 * s], coeffs[4 * s + 1]}, f1 = {coeffs[4 * s + 2], coeffs[4 * s + 3]};
+        const double scale = scale_of(prev);
+        const void *Ein = buf[cur];
+        void *Eout = buf[cur ^ 1];
+        double m = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : m)
+        for (long long cpl = 0; cpl < nout; ++cpl) {
+            const u64 cp = (u64)cpl;
+            u64 p0 = 0;
+            for (int b = 0; b < nb; ++b) p0 ^= T[b * 256 + ((cp >> (8 * b)) & 255)];
+            cd acc = {0.0, 0.0};
+            for (int k = 0; k < nc; ++k) {
+                const u64 p = p0 ^ cb[k];
+                const u64 c = p & mask;
+                const int par = (__builtin_popcountll(c & bt) & 1) ^ o;
+                const int yv = fr ? (int)((p >> ri) & 1) : yy;
+                const cd w = cneg_if(yv ? f1 : f0, yv & par);
+                acc = cadd(acc, cmul(ld(Ein, storage, c), w));
+            }
+            double lm = put(Eout, storage, cp, acc, scale);
+            if (lm != lm) lm = INFINITY;
+            if (lm > m) m = lm;
+        }
+        cur ^= 1;
+        unit_m2[s] = m;
+        if (!(m > 0.0) || !isfinite(m)) {
+            for (int t = s + 1; t < nsteps; ++t) unit_m2[t] = 0.0;
+            return -1 - s;
+        }
+        prev = m;
+    }
+    return cur;
 }
 
 /* total[smap[i] ^ xmask] += env(i) * coef over the 2^r terminal coordinates, env(i) = E[pdep(i, mask)] */

@@ -19,6 +19,11 @@ What each function restates and what pins it:
                      force interval search.
   statevector        dense simulation of the ring circuit (H, S, CZ, T = diag(1, e^{i pi/4})); the
                      mathematical ground truth, pinned by gate identities in the tests.
+  contract_gather,   C restatement (oracle_gather) of the canonical gather executor (dcs_rankwidth/scan_plan.py:
+  gather_block       161-195, dcs_rankwidth/gpu_kernel.py:58-101) with the reference's scale chain
+                     (sampling/conditional_block_gpu.py:92-126), and its block.  Pinned by the dense state
+                     vector (C1, every amplitude) and by block(): the two executors share nothing below the
+                     plan's step list, so their agreement to 1e-12 on several graphs checks both.
   gather_amplitude   numpy restatement of sweep_reference / amplitude_reference (scan_plan.py:161-207).
   graph_amplitude    brute-force graph-state sum (scan_plan.py:6-8) for N <= 20.
 """
@@ -46,6 +51,10 @@ def lib():
         L.oracle_contract.restype = C.c_int
         L.oracle_contract.argtypes = [C.c_int, C.c_int, i32p, u64p, u64p, i32p, i32p, u64p, u64p, dp, i32p, i32p,
                                       C.c_void_p, dp]
+        i64p = P(C.c_int64)
+        L.oracle_gather.restype = C.c_int
+        L.oracle_gather.argtypes = [C.c_int, C.c_int, i32p, i32p, i32p, i32p, i64p, u64p, i32p, i64p, u64p, u64p,
+                                    dp, i32p, i32p, C.c_void_p, C.c_void_p, dp]
         L.oracle_accumulate.restype = None
         L.oracle_accumulate.argtypes = [C.c_int, C.c_void_p, C.c_uint64, C.c_int, u64p, C.c_uint64, C.c_double,
                                         C.c_double, dp]
@@ -119,6 +128,76 @@ def block(bp, x_bits, w=None, storage=1, part=0, parts=1):
         c = coef * bp.slice_scale(r['ls'])
         xm = (r['offset'] >> bp.stop) ^ freeflip
         lib().oracle_accumulate(int(storage), E.ctypes.data_as(C.c_void_p), C.c_uint64(bp.terminal_mask()), k,
+                                _p(smap, C.c_uint64), C.c_uint64(xm), c.real, c.imag, _p(tv, C.c_double))
+    ph = np.ascontiguousarray(bp.phase())
+    U = np.ascontiguousarray(bp.open_unitaries().reshape(-1, 4).view(np.float64))
+    lib().oracle_transform(k, _p(tv, C.c_double), _p(ph, C.c_double), _p(U, C.c_double))
+    return total
+
+
+def gather_tables(bp):
+    """Per-step arrays of the canonical gather plan (byte tables, kernel combinations), concatenated."""
+    import sys
+    sys.path.insert(0, str(HERE.parent / 'dcs245-classical-sampling_amd'))
+    from dcs_amd.scan_plan import byte_tables, kernel_combos
+    steps = bp.steps
+    tabs = [byte_tables(s.Lcols, s.r_out) for s in steps]
+    combs = [np.asarray(kernel_combos(s.kernel), np.uint64) for s in steps]
+    out = {
+        'r_in': np.array([s.r_in for s in steps], np.int32), 'r_out': np.array([s.r_out for s in steps], np.int32),
+        'free': np.array([1 if s.free else 0 for s in steps], np.int32),
+        'beta': np.array([s.beta for s in steps], np.uint64),
+        'nbytes': np.array([t.shape[0] for t in tabs], np.int32),
+        'ncombos': np.array([len(c) for c in combs], np.int32),
+        'tables': np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in tabs]), np.uint64),
+        'combos': np.ascontiguousarray(np.concatenate(combs), np.uint64)}
+    out['tab_off'] = np.concatenate([[0], np.cumsum(out['nbytes'].astype(np.int64) * 256)[:-1]]).astype(np.int64)
+    out['comb_off'] = np.concatenate([[0], np.cumsum(out['ncombos'].astype(np.int64))[:-1]]).astype(np.int64)
+    return out
+
+
+def contract_gather(bp, f, yfix, storage=1, tables=None):
+    """One slice through the canonical gather executor (oracle_gather): dict(env = the final table's
+    first 2^r entries, ls, alive, offset, m2)."""
+    g = gather_tables(bp) if tables is None else tables
+    coeffs, ov, ys, offset = bp.step_parameters(f, yfix)
+    m = len(bp.steps)
+    dt = np.complex128 if storage else np.complex64
+    A = np.zeros(1 << bp.rmax, dt)
+    B = np.zeros(1 << bp.rmax, dt)
+    m2 = np.zeros(m, np.float64)
+    coeffs = np.ascontiguousarray(coeffs)
+    rc = lib().oracle_gather(int(storage), m, _p(g['r_in'], C.c_int32), _p(g['r_out'], C.c_int32),
+                             _p(g['free'], C.c_int32), _p(g['nbytes'], C.c_int32), _p(g['tab_off'], C.c_int64),
+                             _p(g['tables'], C.c_uint64), _p(g['ncombos'], C.c_int32), _p(g['comb_off'], C.c_int64),
+                             _p(g['combos'], C.c_uint64), _p(g['beta'], C.c_uint64), _p(coeffs, C.c_double),
+                             _p(ov, C.c_int32), _p(ys, C.c_int32), A.ctypes.data_as(C.c_void_p),
+                             B.ctypes.data_as(C.c_void_p), _p(m2, C.c_double))
+    ls, alive = frexp_scale_chain(m2)
+    r = len(bp.gather_terminal_basis)
+    env = ((A, B)[rc][:1 << r].copy()) if rc >= 0 else None
+    return {'env': env, 'ls': ls, 'alive': alive and rc >= 0, 'offset': offset, 'm2': m2}
+
+
+def gather_block(bp, x_bits, w=None, storage=1, part=0, parts=1):
+    """Complex128 block by the canonical gather executor (conditional_block_gpu.py:128-177 with
+    _prefix :92-126)."""
+    k = bp.k
+    total = np.zeros(1 << k, np.complex128)
+    tv = total.view(np.float64)
+    smap = np.ascontiguousarray(bp.syndrome_map(bp.gather_terminal_basis))
+    r = len(bp.gather_terminal_basis)
+    g = gather_tables(bp)
+    for _, coef, f, yfix, freeflip in bp.slice_assignments(bp.local_vectors(x_bits, w=w), part, parts):
+        if f is None:
+            continue
+        res = contract_gather(bp, f, yfix, storage, tables=g)
+        if not res['alive']:
+            continue
+        c = coef * bp.slice_scale(res['ls'])
+        xm = (res['offset'] >> bp.stop) ^ freeflip
+        env = np.ascontiguousarray(res['env'])
+        lib().oracle_accumulate(int(storage), env.ctypes.data_as(C.c_void_p), C.c_uint64((1 << r) - 1), k,
                                 _p(smap, C.c_uint64), C.c_uint64(xm), c.real, c.imag, _p(tv, C.c_double))
     ph = np.ascontiguousarray(bp.phase())
     U = np.ascontiguousarray(bp.open_unitaries().reshape(-1, 4).view(np.float64))

@@ -187,6 +187,54 @@ def test_parity_sliced_ring_circuit():
     e.close()
 
 
+@pytest.mark.parametrize('sched', ['steps', 'steps_host'])
+def test_canonical_gather_engine_matches_oracle(c1, w20, sched):
+    """The canonical engine (gather executor): complex128 and complex64 blocks equal the C oracle's
+    gather blocks bit for bit on C1 (all 2^16 amplitudes) and on the width-20 ring circuit."""
+    from dcs_amd.conditional_block import ConditionalBlockGPU
+    for case in (c1, w20):
+        bp = BlockPlan(*case['args'])
+        for dt, st in ((np.complex128, 1), (np.complex64, 0)):
+            print(f'case canonical {sched} {dt.__name__} N={bp.N} rmax={bp.rmax}', flush=True)
+            e = ConditionalBlockGPU(*case['args'])
+            e.alloc(dtype=dt, schedule=sched)
+            v, meta = e.block(case['x'])
+            want = oracle.gather_block(bp, case['x'], storage=st)
+            assert np.array_equal(np.asarray(v), want)
+            assert meta['rmax'] == bp.rmax and meta['terminal_rank'] == len(bp.gather_terminal_basis)
+            e.close()
+    psi_ok = rel(oracle.gather_block(BlockPlan(*c1['args']), c1['x']), c1['ref128'])
+    assert psi_ok < 1e-14
+
+
+def test_canonical_parity_slices_and_prefixes():
+    """Canonical engine on parity slices (Y, Z, X) and on 12-qubit prefix blocks, complex128, bit-exact
+    against the oracle's gather blocks."""
+    from dcs_amd.conditional_block import ConditionalBlockGPU
+    print('case canonical slices/prefixes', flush=True)
+    inst = synthetic_instance(10, 10, 12, seed=2)
+    refs = [v for v in range(10, inst['N']) if inst['graph'][v].any()]
+    nb = [int(u) for u in np.flatnonzero(inst['graph'][refs[2]])]
+    ops = [('Y', refs[0], None), ('Z', refs[1], None), ('X', refs[2], nb[0])]
+    args = (inst['graph'], inst['unitaries'], 10, inst['natural_order'], ops, [0, 3, 6, 8])
+    x = np.array([1, 0, 0, 1, 1, 0, 0, 1, 0, 1], np.uint8)
+    e = ConditionalBlockGPU(*args)
+    e.alloc(dtype=np.complex128)
+    v, meta = e.block(x)
+    assert np.array_equal(np.asarray(v), oracle.gather_block(BlockPlan(*args), x))
+    assert meta['slices_evaluated'] == 8
+    e.close()
+    inst = synthetic_instance(12, 10, 14, seed=1)
+    args = (inst['graph'], inst['unitaries'], 12, inst['natural_order'], [], [9, 2, 7, 11, 4, 0])
+    bp = BlockPlan(*args)
+    e = ConditionalBlockGPU(*args)
+    e.alloc(dtype=np.complex128)
+    for seed in range(3):
+        x = np.random.default_rng(seed).integers(0, 2, 12, dtype=np.uint8)
+        assert np.array_equal(np.asarray(e.block(x)[0]), oracle.gather_block(bp, x))
+    e.close()
+
+
 def test_nonfinite_prefix_raises(c1):
     print('case nonfinite', flush=True)
     e = engine(c1['args'], dtype=np.complex64, schedule='fused')

@@ -111,6 +111,42 @@ def test_inplace_oracle_equals_gather_executor():
     assert abs(abs(val_inplace) * 2 ** (-bp.N_full / 2) - abs(want)) < 1e-13
 
 
+def test_gather_oracle_block_c1_matches_statevector_and_inplace():
+    """The canonical gather executor's block (C1, all 2^16 amplitudes, complex128) equals the in-place
+    block to 1e-14 and the dense state vector up to one global phase; complex64 tables stay within 1e-6."""
+    inst = synthetic_instance(16, 12, 20, 0)
+    bp = BlockPlan(inst['graph'], inst['unitaries'], 16, inst['natural_order'], [], list(range(16)))
+    x = np.zeros(16, np.uint8)
+    g = oracle.gather_block(bp, x)
+    a = oracle.block(bp, x)
+    assert np.max(np.abs(g - a)) / np.max(np.abs(a)) < 1e-14
+    psi = oracle.statevector(inst['ops'], 16)
+    i = int(np.argmax(np.abs(psi)))
+    assert np.max(np.abs(g - (g[i] / psi[i]) * psi)) / np.max(np.abs(g)) < 1e-12
+    g64 = oracle.gather_block(bp, x, storage=0)
+    assert np.max(np.abs(g64 - a)) / np.max(np.abs(a)) < 1e-6
+
+
+def test_gather_oracle_prefix_and_parity_slices_match_inplace():
+    """Prefix blocks (6 open of 12) and a parity-sliced graph (Y, Z and X operations): gather and in-place
+    executors agree to 1e-13."""
+    inst = synthetic_instance(12, 10, 14, 1)
+    bp = BlockPlan(inst['graph'], inst['unitaries'], 12, inst['natural_order'], [], [9, 2, 7, 11, 4, 0])
+    for seed in range(3):
+        x = np.random.default_rng(seed).integers(0, 2, 12, dtype=np.uint8)
+        a = oracle.block(bp, x)
+        assert np.max(np.abs(oracle.gather_block(bp, x) - a)) / np.max(np.abs(a)) < 1e-13
+    inst = synthetic_instance(10, 10, 12, seed=2)
+    refs = [v for v in range(10, inst['N']) if inst['graph'][v].any()]
+    nb = [int(u) for u in np.flatnonzero(inst['graph'][refs[2]])]
+    ops = [('Y', refs[0], None), ('Z', refs[1], None), ('X', refs[2], nb[0])]
+    bp = BlockPlan(inst['graph'], inst['unitaries'], 10, inst['natural_order'], ops, [0, 3, 6, 8])
+    x = np.array([1, 0, 0, 1, 1, 0, 0, 1, 0, 1], np.uint8)
+    a = oracle.block(bp, x)
+    assert len(bp.slices) == 3
+    assert np.max(np.abs(oracle.gather_block(bp, x) - a)) / np.max(np.abs(a)) < 1e-13
+
+
 def test_reference_draw_selftest_vectors():
     """The CDF self-test of sampling_protocol.py:103-111: midpoint of every nonzero interval."""
     from dcs_amd.sampling_protocol import draw_checked
