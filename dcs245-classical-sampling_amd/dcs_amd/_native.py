@@ -19,7 +19,7 @@ MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED, MODE_COMPILED = 0, 1, 2, 3
 EXPORTED = (
     'dcs_last_error', 'dcs_get_device_count', 'dcs_get_device_info', 'dcs_context_create',
     'dcs_context_destroy', 'dcs_synchronize', 'dcs_plan_create', 'dcs_gather_plan_create', 'dcs_plan_destroy',
-    'dcs_plan_compile',
+    'dcs_plan_compile', 'dcs_plan_memory',
     'dcs_plan_units',
     'dcs_plan_schedule', 'dcs_contract', 'dcs_block_zero', 'dcs_block_accumulate', 'dcs_block_transform',
     'dcs_block_read', 'dcs_block_device_pointer', 'dcs_draw', 'dcs_frontier_read', 'dcs_plan_profile',
@@ -83,6 +83,7 @@ def load():
         'dcs_gather_plan_create': (C.c_int, [vp, P(GatherDesc), P(vp)]),
         'dcs_plan_destroy': (None, [vp]),
         'dcs_plan_compile': (C.c_int, [vp, P(dbl)]),
+        'dcs_plan_memory': (C.c_int, [vp, P(u64)]),
         'dcs_plan_units': (C.c_int, [vp, C.c_int, P(C.c_int)]),
         'dcs_plan_schedule': (C.c_int, [vp, C.c_int, C.c_int, P(i32), P(i32), P(i32), P(i32), P(C.c_int)]),
         'dcs_contract': (C.c_int, [vp, C.c_int, C.c_int, P(i32), P(dbl), P(i32), P(i32), P(dbl)]),
@@ -257,6 +258,11 @@ class Plan:
         sec = C.c_double(0.0)
         check(self.lib.dcs_plan_compile(self.handle, C.byref(sec)))
         return sec.value
+
+    def device_bytes(self):
+        b = C.c_uint64(0)
+        check(self.lib.dcs_plan_memory(self.handle, C.byref(b)))
+        return int(b.value)
 
     def units(self, mode):
         u = C.c_int()

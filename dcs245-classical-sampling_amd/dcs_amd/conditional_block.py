@@ -183,6 +183,51 @@ class ConditionalBlockGPU:
                      'work_per_slice': self.work, 'open_vertices': self.open_full,
                      'vector_scale': 'sqrt(2^(n-k)); sum all complex parts before probabilities'}
 
+    def block_many(self, prefixes, w=None, block_indices=None):
+        """Blocks of several prefixes at once: the slices of all of them are contracted `lanes` at a time
+        (one launch covers slices of different prefixes) and each is scattered into its own complex128
+        accumulator.  Every block equals what block() returns for its prefix alone, bit for bit: a slice's
+        contraction does not depend on the lane it runs in, and each block receives its slices in the
+        same order.  Returns ([DeviceBlock], meta)."""
+        if self._plan is None:
+            raise RuntimeError('call alloc() first')
+        if block_indices is None:
+            block_indices = list(range(len(prefixes)))
+        if len(block_indices) != len(prefixes) or len(set(block_indices)) != len(block_indices):
+            raise ValueError('one distinct block index per prefix')
+        if any(not 0 <= b < self._plan.blocks for b in block_indices):
+            raise ValueError(f'block indices must lie in 0..{self._plan.blocks - 1}')
+        start = time.perf_counter()
+        plan = self._plan
+        jobs, counts = [], []
+        for b, x in zip(block_indices, prefixes):
+            js, count = self._slice_jobs(x, w, 0, 1)
+            jobs.extend((b,) + j for j in js)
+            counts.append(count)
+            plan.block_zero(b)
+        launches = 0
+        for i in range(0, len(jobs), self.lanes):
+            batch = jobs[i:i + self.lanes]
+            lanes = np.arange(len(batch), dtype=np.int32)
+            m2 = plan.contract(self.mode, lanes, np.stack([j[2] for j in batch]), np.stack([j[3] for j in batch]),
+                               np.stack([j[4] for j in batch]))
+            launches += 1
+            for lane, (b, coef, _, _, _, offset, freeflip) in enumerate(batch):
+                ls, alive = logscale_from_maxima(m2[lane])
+                if not alive:
+                    continue
+                plan.block_accumulate(b, lane, (offset >> self.stop) ^ freeflip, coef * self.bp.slice_scale(ls))
+        for b in block_indices:
+            plan.block_transform(b)
+        self.ctx.synchronize()
+        return [DeviceBlock(self, b) for b in block_indices], {
+            'seconds': time.perf_counter() - start, 'slices_evaluated': counts, 'contractions': launches,
+            'prefix_dtype': 'complex128' if self.use_fp64 else 'complex64', 'rmax': self.rmax}
+
+    def device_bytes(self):
+        """Device memory this engine holds (frontier(s), plan tables, blocks), as sized at alloc."""
+        return self._plan.device_bytes() if self._plan is not None else 0
+
     def schedule(self):
         return self._plan.schedule(self.mode)
 

@@ -675,6 +675,7 @@ struct dcs_plan {
     std::vector<u64> h_addrv;
     // canonical gather executor (dcs_gather_plan_create): frontier holds the current table
     int executor = 0;  // 0 in-place, 1 gather
+    u64 device_bytes = 0;
     std::vector<GatherArgs> gsteps;
     void *frontier2 = nullptr;
     u64 *d_gtables = nullptr, *d_gcombos = nullptr;
@@ -1475,6 +1476,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     do {                                                                                         \
         if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess)                                   \
             return bail(fail(DCS_ERR_NOMEM, "hipMalloc of %llu bytes failed", (unsigned long long)(bytes))); \
+        p->device_bytes += (u64)(bytes);                                                         \
     } while (0)
 #define HALLOC(ptr, bytes)                                                                       \
     do {                                                                                         \
@@ -1589,6 +1591,7 @@ int dcs_gather_plan_create(dcs_context *ctx, const dcs_gather_desc *d, dcs_plan 
     do {                                                                                         \
         if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess)                                   \
             return bail(fail(DCS_ERR_NOMEM, "hipMalloc of %llu bytes failed", (unsigned long long)(bytes))); \
+        p->device_bytes += (u64)(bytes);                                                         \
     } while (0)
 #define HALLOC(ptr, bytes)                                                                       \
     do {                                                                                         \
@@ -1643,6 +1646,12 @@ int dcs_gather_plan_create(dcs_context *ctx, const dcs_gather_desc *d, dcs_plan 
 int dcs_plan_compile(dcs_plan *p, double *seconds) {
     if (!p) return fail(DCS_ERR_INVALID, "null plan");
     return compile_groups(p, seconds);
+}
+
+int dcs_plan_memory(dcs_plan *p, uint64_t *bytes) {
+    if (!p || !bytes) return fail(DCS_ERR_INVALID, "null argument");
+    *bytes = p->device_bytes;
+    return DCS_OK;
 }
 
 int dcs_plan_units(dcs_plan *p, int mode, int *nunits) {

@@ -1,0 +1,241 @@
+"""Serving loop: continuous batching of sampling requests over several device-resident circuits.
+
+The reference serves samples one at a time per worker process (production/worker.py:106-133: build one
+engine, then for every assigned record contract its block, copy it to the host and draw).  Here one
+process keeps several circuits resident on the GPU and serves a stream of requests:
+
+  * admission -- a circuit is registered only if its frontier(s), plan tables and block accumulators fit
+    the HBM budget left by the circuits already resident; the number of lanes (slice frontiers held at
+    once) is reduced until it fits, and registration fails with MemoryError if even one lane does not;
+  * continuous batching -- requests queue per circuit as they arrive; the loop repeatedly takes the
+    circuit whose oldest waiting request arrived first and serves up to `max_batch` of its waiting
+    requests together (slices of all of them share lane launches, block_many), without waiting for a
+    batch to fill;
+  * every block is bit-identical to serving its request alone, and so is its draw (block_many);
+  * per-request latency (arrival to draw) gives p50 / p99 per circuit.
+
+`poisson_workload` generates seeded arrivals (exponential gaps) over a weighted circuit mix, with
+prefixes and uniforms drawn from PCG64 in the order of the reference protocol (sampling_protocol.py:21-33).
+"""
+from __future__ import annotations
+
+import collections
+import json
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .conditional_block import InplaceConditionalBlockGPU
+
+
+@dataclass
+class Request:
+    rid: int
+    circuit: str
+    x_bits: np.ndarray
+    uniform: float
+    arrival: float                  # seconds after the start of serve()
+    start: float = math.nan         # when its batch started
+    done: float = math.nan          # when its draw completed
+    batch: int = -1
+    index: int = -1                 # drawn block index (A_index)
+    draw: dict = field(default_factory=dict)
+    block: np.ndarray = None        # host copy of the block (serve(keep_blocks=True))
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+def footprint_bytes(bp, dtype, lanes, blocks):
+    """Device bytes an in-place engine will hold: lanes frontiers of 2^maxslots entries, the complex128
+    block accumulators and the per-block tables (syndrome map, phase, draw scratch), plus a fixed
+    allowance for plan tables and the runtime."""
+    esz = np.dtype(dtype).itemsize
+    blen = 1 << bp.k
+    return (lanes << bp.inplace_maxslots) * esz + blen * (16 * blocks + 8 + 8 + 8 + 16) + (64 << 20)
+
+
+class Circuit:
+    def __init__(self, name, engine, lanes, max_batch, nbytes):
+        self.name, self.engine, self.lanes, self.max_batch, self.nbytes = name, engine, lanes, max_batch, nbytes
+        self.served = 0
+
+
+class Server:
+    """Resident circuits and the batching loop on one device."""
+
+    def __init__(self, device=0, hbm_budget=None, reserve=8 << 30, engine_factory=None):
+        self.device = device
+        if hbm_budget is None:
+            hbm_budget = _native.device_info(device)['free_bytes'] - reserve
+        self.budget = int(hbm_budget)
+        self.used = 0
+        self.circuits = collections.OrderedDict()
+        self._factory = engine_factory or InplaceConditionalBlockGPU
+
+    # ---- admission ---------------------------------------------------------------------------
+    def register(self, name, graph, unitaries, n, order, parity_operations, open_vertices, dtype=np.complex64,
+                 lanes=4, max_batch=4, **alloc_kw):
+        if name in self.circuits:
+            raise ValueError(f'circuit {name!r} already registered')
+        engine = self._factory(graph, unitaries, n, order, parity_operations, open_vertices)
+        left = self.budget - self.used
+        lanes = max(1, int(lanes))
+        while lanes >= 1 and footprint_bytes(engine.bp, dtype, lanes, max_batch) > left:
+            lanes //= 2
+        if lanes < 1:
+            need = footprint_bytes(engine.bp, dtype, 1, max_batch)
+            raise MemoryError(f'circuit {name!r} needs {need} device bytes with one lane; {left} of the '
+                              f'{self.budget}-byte budget are left')
+        engine.alloc(dtype=dtype, device=self.device, lanes=lanes, blocks=max_batch, **alloc_kw)
+        nbytes = engine.device_bytes() or footprint_bytes(engine.bp, dtype, lanes, max_batch)
+        self.used += nbytes
+        self.circuits[name] = Circuit(name, engine, lanes, max_batch, nbytes)
+        return self.circuits[name]
+
+    def unregister(self, name):
+        c = self.circuits.pop(name)
+        c.engine.close()
+        self.used -= c.nbytes
+
+    def close(self):
+        for name in list(self.circuits):
+            self.unregister(name)
+
+    # ---- the loop ------------------------------------------------------------------------------
+    def serve(self, requests, realtime=True, keep_blocks=False):
+        """Serve requests (any order; each has an arrival time in seconds from now).  With realtime=False
+        idle gaps are skipped instead of slept through (latencies are then measured on a clock that
+        jumps to the next arrival when nothing is waiting).  Returns the requests, filled in."""
+        for r in requests:
+            if r.circuit not in self.circuits:
+                raise KeyError(f'request {r.rid}: unknown circuit {r.circuit!r}')
+        todo = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        queues = {name: collections.deque() for name in self.circuits}
+        t0 = time.perf_counter()
+        skew = 0.0  # idle time skipped (realtime=False)
+        clock = lambda: time.perf_counter() - t0 + skew
+        i, nbatch = 0, 0
+        while i < len(todo) or any(queues.values()):
+            now = clock()
+            while i < len(todo) and todo[i].arrival <= now:
+                queues[todo[i].circuit].append(todo[i])
+                i += 1
+            waiting = [q for q in queues.values() if q]
+            if not waiting:
+                gap = todo[i].arrival - now
+                if realtime:
+                    time.sleep(max(0.0, gap))
+                else:
+                    skew += max(0.0, gap)
+                continue
+            q = min(waiting, key=lambda q: (q[0].arrival, q[0].rid))
+            c = self.circuits[q[0].circuit]
+            batch = [q.popleft() for _ in range(min(c.max_batch, len(q)))]
+            start = clock()
+            vecs, _ = c.engine.block_many([r.x_bits for r in batch])
+            for r, v in zip(batch, vecs):
+                r.draw = v.draw(r.uniform)
+                if keep_blocks:
+                    r.block = np.asarray(v)
+                r.index = r.draw['A_index']
+                r.start, r.done, r.batch = start, clock(), nbatch
+            c.served += len(batch)
+            nbatch += 1
+        return requests
+
+
+def latency_stats(requests):
+    """Per-circuit and overall count, p50 / p99 latency (ms) and makespan."""
+    out = {}
+    groups = collections.defaultdict(list)
+    for r in requests:
+        groups[r.circuit].append(r.latency)
+        groups['all'].append(r.latency)
+    for name, lat in groups.items():
+        a = np.array(lat) * 1e3
+        out[name] = {'count': len(a), 'p50_ms': float(np.percentile(a, 50)), 'p99_ms': float(np.percentile(a, 99)),
+                     'mean_ms': float(a.mean())}
+    span = max(r.done for r in requests) - min(r.arrival for r in requests)
+    out['all']['samples_per_s'] = len(requests) / span if span > 0 else math.inf
+    return out
+
+
+def poisson_workload(mix, rate, count, seed):
+    """Seeded request stream.  mix: {name: (weight, n, open_vertices)}; rate: mean arrivals per second.
+    Each request draws its circuit by weight, its closed data bits (open ones are 0) and its uniform
+    from one PCG64 stream, as the reference protocol draws prefixes and uniforms (sampling_protocol.py:21-33)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    names = list(mix)
+    w = np.array([mix[k][0] for k in names], np.float64)
+    w /= w.sum()
+    t, reqs = 0.0, []
+    for rid in range(count):
+        t += rng.exponential(1.0 / rate)
+        name = names[int(rng.choice(len(names), p=w))]
+        _, n, opn = mix[name]
+        closed = [q for q in range(n) if q not in set(opn)]
+        x = np.zeros(n, np.uint8)
+        x[closed] = rng.integers(0, 2, len(closed), dtype=np.uint8)
+        reqs.append(Request(rid, name, x, float(rng.random()), t))
+    return reqs
+
+
+# ---- the serving benchmark (C5-style mix on one GPU) --------------------------------------------
+def _synthetic(n, depth, t, seed, k):
+    """A seeded ring circuit with the order rotation of least frontier width and its last k data
+    qubits open."""
+    from .width_planner import best_rotation
+    from .stabilizer import synthetic_instance
+    inst = synthetic_instance(n, depth, t, seed)
+    order, opn, width = best_rotation(inst['graph'], inst['unitaries'], n, inst['natural_order'], k, stride=2)
+    return (inst['graph'], inst['unitaries'], n, order, [], opn), width
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description='Serving benchmark: Poisson mix of resident circuits on one GPU')
+    ap.add_argument('--requests', type=int, default=36)
+    ap.add_argument('--rate', type=float, default=4.0, help='mean arrivals per second')
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--no-ibm', action='store_true')
+    ap.add_argument('--realtime', action='store_true')
+    a = ap.parse_args(argv)
+    from .workloads import load_ibm64
+    srv = Server()
+    mix = {}
+    t0 = time.perf_counter()
+    args32, w32 = _synthetic(32, 40, 120, 0, 16)
+    srv.register('32q/d40/120T', *args32, lanes=8, max_batch=4, schedule='compiled', lds_storage=True)
+    mix['32q/d40/120T'] = (3, 32, args32[5])
+    args40, w40 = _synthetic(40, 48, 180, 0, 16)
+    srv.register('40q/d48/180T', *args40, lanes=4, max_batch=4, schedule='compiled', lds_storage=True)
+    mix['40q/d48/180T'] = (2, 40, args40[5])
+    if not a.no_ibm:
+        w = load_ibm64()
+        srv.register('ibm64', w['graph'], w['unitaries'], 64, w['order'], w['plans']['inplace_k2_fp32']['operations'],
+                     w['open_vertices'], lanes=1, max_batch=2, schedule='compiled', lds_storage=True)
+        mix['ibm64'] = (1, 64, w['open_vertices'])
+    setup = time.perf_counter() - t0
+    # one warm-up request per circuit (first launches, hipRTC modules already compiled at registration)
+    warm = [Request(-1 - j, name, np.zeros(mix[name][1], np.uint8), 0.5, 0.0) for j, name in enumerate(mix)]
+    srv.serve(warm, realtime=False)
+    reqs = poisson_workload(mix, a.rate, a.requests, a.seed)
+    srv.serve(reqs, realtime=a.realtime)
+    stats = latency_stats(reqs)
+    line = {'metric': 'serving: continuous batching over resident circuits, p50/p99 ms per circuit',
+            'requests': a.requests, 'rate_per_s': a.rate, 'seed': a.seed, 'realtime': a.realtime,
+            'setup_s': setup, 'hbm_used_bytes': srv.used, 'hbm_budget_bytes': srv.budget,
+            'circuits': {c.name: {'lanes': c.lanes, 'max_batch': c.max_batch, 'device_bytes': c.nbytes,
+                                  'rmax': c.engine.inplace_rmax, 'served': c.served} for c in srv.circuits.values()},
+            'widths': {'32q/d40/120T': w32, '40q/d48/180T': w40}, 'stats': stats}
+    print(json.dumps(line), flush=True)
+    srv.close()
+
+
+if __name__ == '__main__':
+    main()

@@ -157,6 +157,8 @@ typedef struct {
 
 int dcs_gather_plan_create(dcs_context *ctx, const dcs_gather_desc *desc, dcs_plan **out);
 
+/* Device bytes the plan holds (frontier tables, per-step tables, blocks, scratch). */
+int dcs_plan_memory(dcs_plan *plan, uint64_t *bytes);
 /* Number of scale units (maxima) a contraction in `mode` produces per lane. */
 int dcs_plan_units(dcs_plan *plan, int mode, int *nunits);
 /* Schedule of `mode`: per segment (kind 0 step, 1 resident run, 2 coset group), first step, step

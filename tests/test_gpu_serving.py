@@ -1,0 +1,43 @@
+"""Serving loop on the device: a Poisson mix of three resident circuits; every request's block and draw
+equal serving it alone (bit for bit), and the draws agree with the host draw_checked."""
+import numpy as np
+import pytest
+
+from dcs_amd.sampling_protocol import draw_checked
+from dcs_amd.serving import Server, latency_stats, poisson_workload
+from dcs_amd.stabilizer import synthetic_instance
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mixed_serving_equals_alone():
+    print('case serving mix', flush=True)
+    a = synthetic_instance(12, 10, 14, seed=1)
+    b = synthetic_instance(16, 12, 20, seed=0)
+    c = synthetic_instance(28, 32, 64, seed=0)
+    circuits = {
+        'q12': ((a['graph'], a['unitaries'], 12, a['natural_order'], [], [9, 2, 7, 11, 4, 0]), 4),
+        'q16': ((b['graph'], b['unitaries'], 16, b['natural_order'], [], list(range(10, 16))), 2),
+        'q28': ((c['graph'], c['unitaries'], 28, c['natural_order'], [], list(range(20, 28))), 3),
+    }
+    srv = Server(hbm_budget=32 << 30)
+    mix = {}
+    for name, (args, lanes) in circuits.items():
+        srv.register(name, *args, lanes=lanes, max_batch=3, schedule='compiled', lds_storage=True)
+        mix[name] = (1, args[2], args[5])
+    reqs = poisson_workload(mix, rate=200.0, count=24, seed=5)
+    srv.serve(reqs, realtime=False, keep_blocks=True)
+    st = latency_stats(reqs)
+    print({k: (v['count'], round(v['p50_ms'], 2), round(v['p99_ms'], 2)) for k, v in st.items()}, flush=True)
+    assert st['all']['count'] == 24
+    assert max(r.batch for r in reqs) < 24  # some requests were batched together
+    for r in reqs:
+        eng = srv.circuits[r.circuit].engine
+        vec, _ = eng.block(r.x_bits)
+        alone = np.asarray(vec)
+        assert np.array_equal(alone, r.block), r.rid
+        d = vec.draw(r.uniform)
+        assert d['A_index'] == r.index
+        h = draw_checked(alone, r.uniform)
+        assert h['A_index'] == r.index and h['cdf_lower_numpy'] == d['cdf_lower_numpy']
+    srv.close()

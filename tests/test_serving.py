@@ -1,0 +1,176 @@
+"""Serving loop host logic (admission, continuous batching, workload generator) with a stand-in engine,
+and the width planner (cut ranks, rotations, greedy parity slicing) against brute force and the oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from dcs_amd.block_plan import BlockPlan
+from dcs_amd.serving import Request, Server, footprint_bytes, latency_stats, poisson_workload
+from dcs_amd.stabilizer import synthetic_instance
+from dcs_amd.width_planner import best_rotation, choose_parity_slices, order_cost, prefix_cut_ranks
+from oracle import oracle
+
+
+class _BP:
+    def __init__(self, k, width):
+        self.k, self.inplace_maxslots = k, width
+
+
+class _Vec:
+    def __init__(self, x):
+        self.x = x
+
+    def draw(self, u):
+        return {'A_index': int(self.x.sum()) * 1000 + int(u * 1000)}
+
+
+class FakeEngine:
+    """Stand-in with the engine surface the server uses; records every batch."""
+    log = []
+
+    def __init__(self, graph, unitaries, n, order, ops, opn):
+        self.bp = _BP(len(opn), int(graph))   # `graph` carries the width in these tests
+        self.inplace_rmax = int(graph)
+        self.lanes = self.blocks = None
+
+    def alloc(self, dtype, device, lanes, blocks, **kw):
+        self.lanes, self.blocks, self.dtype = lanes, blocks, dtype
+
+    def device_bytes(self):
+        return footprint_bytes(self.bp, self.dtype, self.lanes, self.blocks)
+
+    def block_many(self, xs):
+        FakeEngine.log.append((self.inplace_rmax, len(xs)))
+        return [_Vec(x) for x in xs], {}
+
+    def close(self):
+        pass
+
+
+def _server(budget):
+    FakeEngine.log = []
+    return Server(hbm_budget=budget, engine_factory=FakeEngine)
+
+
+def test_admission_reduces_lanes_and_refuses():
+    srv = _server(budget=3 * (1 << 30) * 8 + (512 << 20))
+    c = srv.register('a', 30, None, 8, None, [], list(range(4)), lanes=8, max_batch=2)   # 8 GiB per lane
+    assert c.lanes == 2
+    assert srv.used == footprint_bytes(_BP(4, 30), np.complex64, 2, 2)
+    c2 = srv.register('b', 29, None, 8, None, [], list(range(4)), lanes=4, max_batch=2)   # 4 GiB per lane
+    assert c2.lanes == 2 and srv.used <= srv.budget
+    with pytest.raises(MemoryError):
+        srv.register('c', 31, None, 8, None, [], list(range(4)), lanes=1)
+    srv.unregister('a')
+    assert srv.register('c', 31, None, 8, None, [], list(range(4)), lanes=1).lanes == 1
+
+
+def test_continuous_batching_order_and_bounds():
+    srv = _server(budget=1 << 40)
+    srv.register('small', 10, None, 8, None, [], [0, 1], lanes=4, max_batch=3)
+    srv.register('big', 12, None, 8, None, [], [0, 1], lanes=1, max_batch=2)
+    x = np.zeros(8, np.uint8)
+    reqs = [Request(0, 'big', x, .1, 0.0), Request(1, 'small', x, .2, 0.001), Request(2, 'small', x, .3, 0.002),
+            Request(3, 'small', x, .4, 0.003), Request(4, 'small', x, .5, 0.004), Request(5, 'big', x, .6, 0.005)]
+    srv.serve(reqs, realtime=False)
+    assert all(r.index >= 0 and r.done >= r.arrival for r in reqs)
+    assert sorted({r.batch for r in reqs}) == list(range(len(FakeEngine.log)))
+    for b in range(len(FakeEngine.log)):
+        members = [r for r in reqs if r.batch == b]
+        assert len({r.circuit for r in members}) == 1
+        limit = 3 if members[0].circuit == 'small' else 2
+        assert len(members) <= limit
+    # FIFO within a circuit: batch numbers never decrease with arrival
+    for name in ('small', 'big'):
+        seq = [r.batch for r in sorted(reqs, key=lambda r: r.arrival) if r.circuit == name]
+        assert seq == sorted(seq)
+    st = latency_stats(reqs)
+    assert st['all']['count'] == 6 and st['small']['count'] == 4 and st['all']['p99_ms'] >= st['all']['p50_ms']
+
+
+def test_poisson_workload_is_seeded_and_weighted():
+    mix = {'a': (3, 10, [8, 9]), 'b': (1, 12, [0])}
+    r1 = poisson_workload(mix, rate=50.0, count=400, seed=7)
+    r2 = poisson_workload(mix, rate=50.0, count=400, seed=7)
+    assert [(r.circuit, r.uniform, r.arrival, r.x_bits.tobytes()) for r in r1] == \
+           [(r.circuit, r.uniform, r.arrival, r.x_bits.tobytes()) for r in r2]
+    frac = sum(r.circuit == 'a' for r in r1) / len(r1)
+    assert 0.68 < frac < 0.82
+    gaps = np.diff([0.0] + [r.arrival for r in r1])
+    assert abs(gaps.mean() - 1 / 50.0) < 0.2 / 50.0
+    assert all(r.x_bits[8] == 0 and r.x_bits[9] == 0 for r in r1 if r.circuit == 'a')
+
+
+# ---- width planner ---------------------------------------------------------------------------------
+def _gf2rank(M):
+    M = (M % 2).astype(np.uint8)
+    r = 0
+    for c in range(M.shape[1]):
+        p = [i for i in range(r, M.shape[0]) if M[i, c]]
+        if not p:
+            continue
+        M[[r, p[0]]] = M[[p[0], r]]
+        for i in range(M.shape[0]):
+            if i != r and M[i, c]:
+                M[i] ^= M[r]
+        r += 1
+        if r == M.shape[0]:
+            break
+    return r
+
+
+def test_prefix_cut_ranks_match_brute_force():
+    rng = np.random.default_rng(3)
+    for _ in range(6):
+        N = 15
+        A = np.triu((rng.random((N, N)) < 0.3).astype(np.uint8), 1)
+        A = A | A.T
+        order = [int(v) for v in rng.permutation(N)[:13]]
+        want = [_gf2rank(A[np.ix_(order[:i], order[i:])]) for i in range(1, len(order))]
+        assert list(prefix_cut_ranks(A, order)) == want
+
+
+def test_ibm_order_cut_rank_is_36(ibm64):
+    """The unsliced IBM graph has cut rank 36 along its frozen order."""
+    assert int(prefix_cut_ranks(ibm64['graph'], ibm64['order']).max()) == 36
+
+
+def test_best_rotation_not_wider_than_natural():
+    inst = synthetic_instance(16, 12, 20, seed=0)
+    order, opn, w = best_rotation(inst['graph'], inst['unitaries'], 16, inst['natural_order'], 6)
+    nat_data = [v for v in inst['natural_order'] if v < 16]
+    w_nat = BlockPlan(inst['graph'], inst['unitaries'], 16, inst['natural_order'], [], nat_data[-6:]).inplace_maxslots
+    assert w <= w_nat and len(opn) == 6 and sorted(order) == sorted(inst['natural_order'])
+
+
+def test_greedy_parity_slices_lower_width_and_keep_the_block():
+    """Two greedy slices on a 10-qubit ring: the max cut rank does not grow, and the sliced block equals
+    the unsliced one up to one global phase (oracle)."""
+    inst = synthetic_instance(10, 10, 14, seed=3)
+    n, N = 10, inst['N']
+    edges = inst['edges']
+    refs = set(range(n, N))
+    ops, removed = choose_parity_slices(N, edges, inst['natural_order'], 2, seed=0, allowed=refs)
+    assert len(ops) == 2 and len(set(removed)) == 2 and set(removed) <= refs
+    A = inst['graph']
+    before = order_cost(A, inst['natural_order'])[0]
+    opn = [0, 4, 7]
+    x = np.array([1, 0, 1, 1, 0, 1, 0, 0, 1, 1], np.uint8)
+    full = oracle.block(BlockPlan(inst['graph'], inst['unitaries'], n, inst['natural_order'], [], opn), x)
+    bp = BlockPlan(inst['graph'], inst['unitaries'], n, inst['natural_order'], ops, opn)
+    sl = oracle.block(bp, x)
+    i = int(np.argmax(np.abs(full)))
+    assert np.max(np.abs(sl - (sl[i] / full[i]) * full)) / np.max(np.abs(full)) < 1e-12
+    assert bp.rmax <= before + 1
+
+
+def test_greedy_slicing_reproduces_the_frozen_ibm_k5_operations(ibm64):
+    """The frozen canonical_k5_fp64 plan of the 64-qubit graph is the greedy's choice with seed 0 and
+    k = 5 along the frozen order (inputs/plans.json)."""
+    A = ibm64['graph']
+    edges = [(int(a), int(b)) for a, b in zip(*np.nonzero(np.triu(A, 1)))]
+    ops, _ = choose_parity_slices(len(A), edges, ibm64['order'], 5, seed=0)
+    want = [tuple(None if x is None else (x if isinstance(x, str) else int(x)) for x in op)
+            for op in ibm64['plans']['canonical_k5_fp64']['operations']]
+    assert ops == want
