@@ -243,9 +243,9 @@ class ConditionalBlockGPU:
 class InplaceConditionalBlockGPU(ConditionalBlockGPU):
     """Reference-compatible in-place engine (conditional_block_inplace_gpu.py:17-38, 194-215)."""
 
-    def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='fused',
+    def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
               fuse_max_dim=6, resident_bits=12, tile_log2=11, blocks=1, exact_rounding=False,
-              pass_max_dim=3, lds_storage=False):
+              pass_max_dim=3, lds_storage=True):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
