@@ -1,8 +1,11 @@
 """Write ibm_qat245_64q.npz: the frozen 64-qubit QAT #245 injected graph and its slice plans.
 
 Source files (read-only, in the reference checkout): inputs/injected_graph.npz (graph u8[378,378],
-unitaries c128[378,2,2], order, ring, natural_order) and inputs/plans.json (open vertices and the
-parity-operation lists of every plan).  The adjacency is stored bit-packed; nothing else changes.
+unitaries c128[378,2,2], order, ring, natural_order), inputs/plans.json (open vertices and the
+parity-operation lists of every plan) and protocol.json (the frozen sampling protocol's parameters:
+generation seed, count, free / closed vertices, mixture weight, worker count, plan names, the audit
+selection; not its 1,389 per-record RNG states, which make_protocol regenerates from the seed).  The
+adjacency is stored bit-packed; nothing else changes.
 
 Usage: python extract_ibm.py /root/reference
 """
@@ -17,6 +20,14 @@ def main(ref):
     ref = Path(ref)
     src = np.load(ref / 'inputs/injected_graph.npz', allow_pickle=False)
     plans = json.loads((ref / 'inputs/plans.json').read_text())
+    prot = json.loads((ref / 'protocol.json').read_text())
+    g = prot['generation']
+    protocol = {
+        'schema_version': prot['schema_version'], 'engine': prot['engine'], 'worker_count': prot['worker_count'],
+        'generation': {k: g[k] for k in ('seed', 'count', 'n', 'free_vertices', 'closed_vertices', 'mixture_weight',
+                                         'rng', 'numpy_version')},
+        'generation_plan': prot['generation_plan'], 'reference_plan': prot['reference_plan'],
+        'audit': prot['audit'], 'anchor_input_data_bits': prot['anchor']['input_data_bits'], 'law': prot['law']}
     graph = src['graph'].astype(np.uint8)
     assert (graph == graph.T).all() and not graph.diagonal().any()
     out = {
@@ -28,6 +39,7 @@ def main(ref):
         'natural_order': src['natural_order'].astype(np.int64),
         'open_vertices': np.asarray(plans['open_vertices'], np.int64),
         'plans_json': np.frombuffer(json.dumps(plans['plans']).encode(), np.uint8),
+        'protocol_json': np.frombuffer(json.dumps(protocol).encode(), np.uint8),
     }
     dst = Path(__file__).resolve().parent / 'ibm_qat245_64q.npz'
     np.savez_compressed(dst, **out)

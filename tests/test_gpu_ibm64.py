@@ -9,6 +9,8 @@
   to 1e-5; amplitudes at the drawn index agree with the c128 scores to 1e-5 relative up to one global
   phase per plan (the k2 and k3 parity operations differ by a pivot, whose local-complementation
   identity holds up to a phase; the reference's audits compare |overlap| for the same reason)."""
+import json
+
 import numpy as np
 import pytest
 
@@ -77,3 +79,33 @@ def test_ibm64_c64_k2_draws_match_r1(ibm64, records, sched, lds):
         assert abs(ratio - phase) <= 1e-5
         assert abs(d['scaled_mass_cdf'] - s['mass']) <= 1e-5
     e.close()
+
+
+def test_ibm64_driver_worker_with_audit_matches_r1(ibm64, records, tmp_path):
+    """The production driver on the first preselected audit prefix: complex64 generation (k2, compiled
+    schedule), draw and records, complex128 reference (k3), the 53-bit CDF law witness; the record
+    verifier passes on the directory, and draw index, masses, witness and loss equal the r1 records."""
+    from dcs_amd.driver import run_worker, verify_worker_dir
+    sid = min(int(k) for k in records['audits'])
+    s, a = records['samples'][str(sid)], records['audits'][str(sid)]
+    worker = sid % ibm64['protocol']['worker_count']
+    print(f'case ibm64 driver worker {worker} sample {sid} with audit', flush=True)
+    out = tmp_path / f'worker_{worker}'
+    summary = run_worker(out, worker, sample_ids=[sid], workload=ibm64)
+    assert summary['audit_ids'] == [sid]
+    res = verify_worker_dir(out, workload=ibm64)
+    print(json.dumps({k: res[k] for k in ('passed', 'checks')}), res['failures'][:3], flush=True)
+    assert res['passed'], res['failures'][:5]
+    row = json.loads((out / 'samples.jsonl').read_text().splitlines()[0])
+    assert row['input_data_bits'] == s['bits'] and row['draw_uniform'] == s['u']
+    if s['margin'] >= TIE:
+        assert row['draw']['A_index'] == s['A_index']
+    assert abs(row['draw']['scaled_mass_fsum'] - s['mass']) <= 1e-5
+    audit = json.loads((out / 'audits.jsonl').read_text().splitlines()[0])
+    w = audit['witness']
+    assert abs(w['reference_scaled_mass'] - a['ref_mass']) <= 1e-10
+    assert abs(w['approximate_input_mass'] - 1.0) <= 1e-12
+    assert abs(w['uncapped_overlap_witness'] - a['witness']) <= 2e-6
+    assert abs(w['loss'] - a['loss']) <= 2e-6
+    hw = json.loads((out / 'hardware.json').read_text())
+    assert hw['arch'].startswith('gfx950')
