@@ -7,7 +7,8 @@ every slice contraction, the coherent slice sum, the Walsh-Hadamard / phase / Cl
 device CDF draw.  Inputs (graph, plan, frontier buffer) are resident before the timed region.
 
 Fast path (value): complex64 storage, plan inplace_k2_fp32 (4 slices, rmax 34, 128 GiB frontier),
-compiled schedule (LDS-resident runs + one generated kernel per coset group, complex64 LDS tiles).
+compiled schedule (LDS-resident runs + one generated kernel per coset group, complex64 LDS tiles;
+coset groups of dimension <= 8 on 2^12-entry tiles with the fewest-HBM-bytes group boundaries).
 Plain path (reported beside it): complex128
 storage, plan inplace_k3_fp64 (8 slices, rmax 33), one launch per step and a host read of every
 maximum, one prefix at a time -- the reference method.
@@ -92,7 +93,7 @@ def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False, **allo
         vec, _ = e.block(x)
         vec.draw(u)
         prof = e._plan.profile(False)
-    info = {'engine': e, 'elapsed': elapsed, 'lat': lat, 'setup': setup, 'draws': draws, 'profile': prof}
+    info = {'engine': e, 'options': dict(e.schedule_options), 'elapsed': elapsed, 'lat': lat, 'setup': setup, 'draws': draws, 'profile': prof}
     return info
 
 
@@ -204,7 +205,10 @@ def main():
     tf = ROOT / 'profiles' / ('coset_groups_ibm64_k2_c64_traffic.json' if a.schedule == 'compiled'
                               else 'k_coset_ibm64_k2_c64_traffic.json')
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get('hbm_bytes_per_launch')
+        rec = json.loads(tf.read_text())
+        # counter-derived bytes belong to one schedule: use them only for the same launches and bytes
+        if rec.get('launches_per_sample') == int(launches[2]) and rec.get('algorithmic_bytes_per_sample') == coset_bytes:
+            traffic = rec.get('hbm_bytes_per_launch')
     lat = np.array(fast['lat']) * 1e3
     line = {
         'metric': 'samples/s @64q 2^20 block: batched c64 vs plain c128, HBM GB/s; p50/p99 ms',
@@ -216,6 +220,8 @@ def main():
         'config': {'workload': 'IBM QAT #245 64q/d73/314T injected graph (N=378), plan inplace_k2_fp32 '
                                '(4 parity slices, rmax 34, 128 GiB c64 frontier), 2^20 block, 44-bit prefixes',
                    'schedule': a.schedule, 'lds_tiles': ('fp64' if a.lds_fp64 else 'storage') if a.schedule == 'compiled' else None,
+                   'fusion': {k: fast['options'][k] for k in ('fuse_max_dim', 'tile_log2', 'pass_max_dim', 'group_policy',
+                                                               'lds_buffers')},
                    'block': 1 << 20},
         'setup_s': fast['setup'],
         'effective_hbm_GBps': unfused_bytes * sps / 1e9,

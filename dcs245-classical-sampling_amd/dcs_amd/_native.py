@@ -41,7 +41,8 @@ class PlanDesc(C.Structure):
                 ('syndrome_map', C.POINTER(C.c_uint64)), ('phase', C.POINTER(C.c_double)),
                 ('open_unitaries', C.POINTER(C.c_double)), ('blocks', C.c_int32),
                 ('fuse_max_dim', C.c_int32), ('resident_bits', C.c_int32), ('tile_log2', C.c_int32),
-                ('exact_rounding', C.c_int32), ('pass_max_dim', C.c_int32), ('lds_storage', C.c_int32)]
+                ('exact_rounding', C.c_int32), ('pass_max_dim', C.c_int32), ('lds_storage', C.c_int32),
+                ('group_policy', C.c_int32), ('lds_buffers', C.c_int32)]
 
 
 class GatherDesc(C.Structure):
@@ -173,7 +174,7 @@ class Plan:
 
     def __init__(self, ctx, table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, phase,
                  open_unitaries, blocks=1, fuse_max_dim=6, resident_bits=12, tile_log2=11, exact_rounding=False,
-                 pass_max_dim=3, lds_storage=False):
+                 pass_max_dim=3, lds_storage=False, group_policy=0, lds_buffers=2):
         self.ctx, self.lib = ctx, ctx.lib
         n = len(table['kind'])
         keep = {}
@@ -203,6 +204,7 @@ class Plan:
         d.exact_rounding = int(bool(exact_rounding))
         d.pass_max_dim = int(pass_max_dim)
         d.lds_storage = int(bool(lds_storage))
+        d.group_policy, d.lds_buffers = int(group_policy), int(lds_buffers)
         h = C.c_void_p()
         check(self.lib.dcs_plan_create(ctx.handle, C.byref(d), C.byref(h)))
         self.handle = h

@@ -244,8 +244,8 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
     """Reference-compatible in-place engine (conditional_block_inplace_gpu.py:17-38, 194-215)."""
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
-              fuse_max_dim=6, resident_bits=12, tile_log2=11, blocks=1, exact_rounding=False,
-              pass_max_dim=3, lds_storage=True):
+              fuse_max_dim=8, resident_bits=12, tile_log2=12, blocks=1, exact_rounding=False,
+              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=1):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
@@ -259,6 +259,13 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
         lds_storage  compiled schedule: hold coset tiles in LDS in storage precision (half the LDS of
                    FP64 tiles for complex64, so twice the workgroups per CU); values are then rounded
                    to storage once per register pass
+        fuse_max_dim, tile_log2, pass_max_dim  largest coset dimension of a fused group, log2 entries
+                   of its LDS tile (11, 12; 13 compiled only), span of one register pass (3, 4); the
+                   defaults (8, 12, 3) are the fastest measured on the 64-qubit plans
+        group_policy  0: greedy maximal coset groups; 1 (default): the group boundaries with the fewest
+                   HBM bytes (dynamic programming)
+        lds_buffers  compiled schedule: 1 (default) one LDS tile per workgroup, the next tile waiting in
+                   registers; 2 double-buffered tiles
         The legacy fp64_kernel_path argument is accepted and ignored, as in the reference."""
         dt = np.dtype(dtype)
         if dt not in (np.dtype(np.complex64), np.dtype(np.complex128)):
@@ -279,7 +286,12 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                                   bp.terminal_mask(), bp.syndrome_map(), bp.phase(), bp.open_unitaries(),
                                   blocks=blocks, fuse_max_dim=fuse_max_dim, resident_bits=resident_bits,
                                   tile_log2=tile_log2, exact_rounding=exact_rounding,
-                                  pass_max_dim=pass_max_dim, lds_storage=lds_storage)
+                                  pass_max_dim=pass_max_dim, lds_storage=lds_storage,
+                                  group_policy=group_policy, lds_buffers=lds_buffers)
+        self.schedule_options = {'schedule': schedule, 'fuse_max_dim': fuse_max_dim, 'tile_log2': tile_log2,
+                                 'pass_max_dim': pass_max_dim, 'resident_bits': resident_bits,
+                                 'group_policy': group_policy, 'lds_buffers': lds_buffers,
+                                 'lds_storage': bool(lds_storage), 'exact_rounding': bool(exact_rounding)}
         self.compile_seconds = self._plan.compile() if self.mode == _native.MODE_COMPILED else 0.0
         self.buf = True
 

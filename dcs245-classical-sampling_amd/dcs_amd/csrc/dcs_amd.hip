@@ -659,7 +659,7 @@ struct PendingEvent {
 struct dcs_plan {
     dcs_context *ctx;
     int storage, maxslots, lanes, nsteps, k, blocks;
-    int fuse_max_dim, resident_bits, tile_log2, exact_rounding, pass_max_dim, lds_storage;
+    int fuse_max_dim, resident_bits, tile_log2, exact_rounding, pass_max_dim, lds_storage, group_policy, lds_buffers;
     size_t esz;
     u64 lane_stride;
     u64 terminal_mask;
@@ -683,6 +683,7 @@ struct dcs_plan {
     bool compiled = false;
     std::vector<hipFunction_t> group_fn;
     std::vector<unsigned> group_grid;
+    std::vector<int> group_threads;
     int units_steps, units_fused;
     // device memory
     void *frontier = nullptr;
@@ -796,6 +797,87 @@ bool step_is_resident(const StepHost &s, int rb) {
     return !(s.in_mask & ~lim) && !(s.out_mask & ~lim);
 }
 
+// Ends of the coset groups that may start at step s: every e such that steps [s, e) span an orbit-mask
+// subspace of dimension <= D (a single step always qualifies), every slot grown inside the run is a
+// pivot of that subspace, and every slot dead at the run's start or end is one too (the coset
+// kernels zero-fill / skip exactly those coordinates).
+std::vector<int> group_ends_from(const dcs_plan *p, int s, int D) {
+    std::vector<int> ends;
+    std::vector<u64> V;
+    u64 Aun = p->steps[s].in_mask;
+    for (int e = s; e < p->nsteps && !step_is_resident(p->steps[e], p->resident_bits); ++e) {
+        for (u64 m : step_masks(p->steps[e])) rref_insert(V, m);
+        if ((int)V.size() > D && e > s) break;
+        if (p->steps[e].kind == 0) Aun |= 1ull << p->steps[e].pivot;
+        u64 piv = 0;
+        for (u64 v : V) piv |= 1ull << top_bit(v);
+        const u64 dead = (Aun & ~p->steps[s].in_mask) | (Aun & ~p->steps[e].out_mask);
+        if (!(piv & ~Aun) && !(dead & ~piv)) ends.push_back(e + 1);
+    }
+    return ends;
+}
+
+// Group boundaries over the non-resident runs of the plan.  Policy 0: greedy maximal groups (extend
+// while the span stays <= D).  Policy 1: the partition with the fewest HBM bytes, by dynamic
+// programming over group ends: a group reads its live entries once and writes them once, so its cost
+// is (2^|live before| + 2^|live after|) entries plus a fixed launch allowance of 2^20 entries.
+void choose_groups(const dcs_plan *p, int D, std::vector<int> &group_end) {
+    const int n = p->nsteps;
+    group_end.assign(n, -1);
+    int s = 0;
+    while (s < n) {
+        if (step_is_resident(p->steps[s], p->resident_bits)) {
+            ++s;
+            continue;
+        }
+        int b = s;
+        while (b < n && !step_is_resident(p->steps[b], p->resident_bits)) ++b;
+        auto greedy = [&]() {
+            for (int i = s; i < b;) {
+                std::vector<u64> V;
+                int e = i + 1;
+                for (int j = i; j < b; ++j) {
+                    for (u64 m : step_masks(p->steps[j])) rref_insert(V, m);
+                    if ((int)V.size() > D && j > i) break;
+                    e = j + 1;
+                }
+                group_end[i] = e;
+                i = e;
+            }
+        };
+        if (p->group_policy == 0) {
+            greedy();
+        } else {
+            const double inf = 1e300;
+            std::vector<double> best(b - s + 1, inf);
+            std::vector<int> from(b - s + 1, -1);
+            best[0] = 0;
+            auto live = [&](u64 m) { return std::ldexp(1.0, popc(m)); };
+            for (int i = s; i < b; ++i) {
+                if (best[i - s] >= inf) continue;
+                for (int e : group_ends_from(p, i, D)) {
+                    const double c = best[i - s] + live(p->steps[i].in_mask) + live(p->steps[e - 1].out_mask) +
+                                     std::ldexp(1.0, 20);
+                    if (c < best[e - s]) {
+                        best[e - s] = c;
+                        from[e - s] = i;
+                    }
+                }
+            }
+            if (best[b - s] >= inf) {
+                greedy();  // no admissible partition: the greedy groups report the offending step
+            } else {
+                for (int e = b; e > s;) {
+                    const int i = from[e - s];
+                    group_end[i] = e;
+                    e = i;
+                }
+            }
+        }
+        s = b;
+    }
+}
+
 int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<PassDev> &passes,
                     std::vector<PStepDev> &psteps, std::vector<u64> &addrv) {
     const int n = p->nsteps;
@@ -807,6 +889,8 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
     int unit = 0;
     const int D = std::max(1, std::min(kMaxDimCompiled, p->fuse_max_dim));
     const int TL = p->tile_log2;
+    std::vector<int> group_end;
+    if (p->fuse_max_dim > 0) choose_groups(p, D, group_end);
     int s = 0;
     while (s < n) {
         if (step_is_resident(p->steps[s], p->resident_bits)) {
@@ -823,16 +907,13 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
             ++s;
             continue;
         }
+        const int e = group_end[s];
+        if (e <= s) return fail(DCS_ERR_INVALID, "no group boundary chosen at step %d", s);
         std::vector<u64> V;
         u64 Aun = p->steps[s].in_mask;
-        int e = s;
-        while (e < n && !step_is_resident(p->steps[e], p->resident_bits)) {
-            std::vector<u64> V2 = V;
-            for (u64 m : step_masks(p->steps[e])) rref_insert(V2, m);
-            if ((int)V2.size() > D && e > s) break;
-            V = V2;
-            if (p->steps[e].kind == 0) Aun |= 1ull << p->steps[e].pivot;
-            ++e;
+        for (int j = s; j < e; ++j) {
+            for (u64 m : step_masks(p->steps[j])) rref_insert(V, m);
+            if (p->steps[j].kind == 0) Aun |= 1ull << p->steps[j].pivot;
         }
         const int d = (int)V.size();
         u64 piv = 0;
@@ -900,82 +981,101 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
         }
         // Split the group into register passes: consecutive steps whose t-space masks stay linearly
         // independent, at most three masks per pass.
+        // m: t-space masks of the step (K or Ka); r: t-coordinate of each mask's slot pivot, the bit that
+        // is zero on the element playing "c".  With the always-live-first coordinate order this need not
+        // be the mask's highest t-bit.
         struct TStep {
             int j;
             uint32_t m[2];
+            int r[2];
             int nm;
         };
         std::vector<TStep> ts;
         for (int j = s; j < e; ++j) {
             const StepHost &st = p->steps[j];
-            TStep t{j, {0u, 0u}, 0};
+            TStep t{j, {0u, 0u}, {-1, -1}, 0};
             uint32_t c = 0;
+            auto add_mask = [&](uint32_t m, int slot) -> bool {
+                const int r = tindex(slot);
+                if (r < 0 || !((m >> r) & 1)) return false;
+                t.m[t.nm] = m;
+                t.r[t.nm++] = r;
+                return true;
+            };
             if (st.kind == 0) {
                 const int i = tindex(st.pivot);
-                if (i < 0) return fail(DCS_ERR_INVALID, "step %d: grow slot outside its group subspace", j);
-                t.m[t.nm++] = 1u << i;
+                if (i < 0 || !add_mask(1u << i, st.pivot))
+                    return fail(DCS_ERR_INVALID, "step %d: grow slot outside its group subspace", j);
             } else if (st.kind == 1) {
-                if (!tcoords(st.Ka, c)) return fail(DCS_ERR_INVALID, "step %d: Ka outside its group subspace", j);
-                t.m[t.nm++] = c;
+                if (!tcoords(st.Ka, c) || !add_mask(c, st.pivot_a))
+                    return fail(DCS_ERR_INVALID, "step %d: Ka outside its group subspace", j);
             } else if (st.kind == 3 || st.kind == 5 || st.kind == 6 || st.kind == 7) {
-                if (!tcoords(st.K, c)) return fail(DCS_ERR_INVALID, "step %d: K outside its group subspace", j);
-                t.m[t.nm++] = c;
+                if (!tcoords(st.K, c) || !add_mask(c, st.pivot))
+                    return fail(DCS_ERR_INVALID, "step %d: K outside its group subspace", j);
                 if (st.kind == 6) {
-                    if (!tcoords(st.Ka, c)) return fail(DCS_ERR_INVALID, "step %d: Ka outside its group subspace", j);
-                    t.m[t.nm++] = c;
+                    if (!tcoords(st.Ka, c) || !add_mask(c, st.pivot_a))
+                        return fail(DCS_ERR_INVALID, "step %d: Ka outside its group subspace", j);
                 }
             }
             ts.push_back(t);
         }
         auto top32 = [](uint32_t x) { return 31 - __builtin_clz(x); };
-        // Pass state: basis vectors (insertion order) and an echelon copy keyed by top bit that
-        // remembers which basis vectors each reduced vector combines.
+        // Pass state: basis vectors (insertion order) and a fully reduced copy in which every reduced
+        // vector owns a pivot bit that no other reduced vector has, remembering which basis vectors it
+        // combines.  Orbit representatives are zero on the pivots, so a step's role coordinate (the
+        // t-bit that is zero on its "c" element) must be a pivot for the role to be a compile-time
+        // property of the element index: pivots are chosen as the role coordinates.
+        struct Red {
+            uint32_t vec, combo;
+            int piv;
+        };
         struct PassState {
             std::vector<uint32_t> basis;
-            std::vector<std::pair<uint32_t, uint32_t>> red;  // (reduced vector, combination of basis)
+            std::vector<Red> red;
+            bool is_pivot(int b) const {
+                for (const Red &rc : red)
+                    if (rc.piv == b) return true;
+                return false;
+            }
         };
-        auto coords = [&](const PassState &P, uint32_t v, uint32_t &combo) -> bool {
+        auto reduce = [&](const PassState &P, uint32_t v, uint32_t &combo) -> uint32_t {
             combo = 0;
-            bool progress = true;
-            while (v && progress) {
-                progress = false;
-                for (const auto &rc : P.red)
-                    if ((v >> top32(rc.first)) & 1) {
-                        v ^= rc.first;
-                        combo ^= rc.second;
-                        progress = true;
-                    }
-            }
-            return v == 0;
+            for (const Red &rc : P.red)
+                if ((v >> rc.piv) & 1) {
+                    v ^= rc.vec;
+                    combo ^= rc.combo;
+                }
+            return v;
         };
-        auto add_basis = [&](PassState &P, uint32_t v) -> int {
-            uint32_t combo = 0;
-            uint32_t r = v;
-            bool progress = true;
-            while (r && progress) {
-                progress = false;
-                for (const auto &rc : P.red)
-                    if ((r >> top32(rc.first)) & 1) {
-                        r ^= rc.first;
-                        combo ^= rc.second;
-                        progress = true;
-                    }
-            }
+        // Express mask m (role coordinate r) in the pass basis, adding it as a new basis vector when it
+        // is outside the span; false when r cannot be a pivot or the span would exceed pass_max_dim.
+        auto express = [&](PassState &P, uint32_t m, int r, uint32_t &c) -> bool {
+            const uint32_t rem = reduce(P, m, c);
+            if (!rem) return P.is_pivot(r);
+            if ((int)P.basis.size() >= p->pass_max_dim) return false;
+            int piv;
+            if ((rem >> r) & 1) piv = r;
+            else if (P.is_pivot(r)) piv = top32(rem);
+            else return false;
             const int idx = (int)P.basis.size();
-            P.basis.push_back(v);
-            P.red.push_back({r, combo | (1u << idx)});
-            return idx;
+            const uint32_t combo = c | (1u << idx);
+            for (Red &rc : P.red)
+                if ((rc.vec >> piv) & 1) {
+                    rc.vec ^= rem;
+                    rc.combo ^= combo;
+                }
+            P.basis.push_back(m);
+            P.red.push_back({rem, combo, piv});
+            c = 1u << idx;
+            return true;
         };
-        // Place one step into a pass state; returns false when it does not fit (span > kPassMax, or a
-        // quad whose K / Ka are not single basis vectors).
+        // Place one step into a pass state; returns false when it does not fit (span > pass_max_dim, a
+        // role coordinate that cannot be a pivot, or a quad whose K / Ka are not single basis vectors).
         auto place = [&](PassState &P, const TStep &t, uint32_t &pat, uint32_t &pat2) -> bool {
             pat = pat2 = 0;
             for (int q = 0; q < t.nm; ++q) {
                 uint32_t c = 0;
-                if (!coords(P, t.m[q], c)) {
-                    if ((int)P.basis.size() >= p->pass_max_dim) return false;
-                    c = 1u << add_basis(P, t.m[q]);
-                }
+                if (!express(P, t.m[q], t.r[q], c)) return false;
                 if (t.nm == 2 && __builtin_popcount(c) != 1) return false;
                 (q == 0 ? pat : pat2) = c;
             }
@@ -1004,7 +1104,7 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
             ps.nsteps = (int)(k1 - k0);
             ps.first_pstep = (int)psteps.size();
             std::vector<int> piv;
-            for (const auto &rc : P.red) piv.push_back(top32(rc.first));
+            for (const Red &rc : P.red) piv.push_back(rc.piv);
             std::sort(piv.begin(), piv.end());
             for (size_t i = 0; i < P.basis.size(); ++i) {
                 ps.piv[i] = piv[i];
@@ -1029,10 +1129,10 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
                     if (popc(V[i] & st.beta) & 1) q.bvec |= 1u << i;
                 for (size_t l = 0; l < P.basis.size(); ++l)
                     if (__builtin_popcount(P.basis[l] & q.bvec) & 1) q.lbv |= 1u << l;
-                if (ts[k].nm >= 1) q.pm = rolemask(top32(ts[k].m[0]));
+                if (ts[k].nm >= 1) q.pm = rolemask(ts[k].r[0]);
                 if (ts[k].nm == 2) {
-                    q.pma = rolemask(top32(ts[k].m[1]));
-                    if ((ts[k].m[1] >> top32(ts[k].m[0])) & 1)
+                    q.pma = rolemask(ts[k].r[1]);
+                    if ((ts[k].m[1] >> ts[k].r[0]) & 1)
                         return fail(DCS_ERR_INVALID, "step %d: Ka holds the K pivot", ts[k].j);
                 }
                 {
@@ -1139,7 +1239,7 @@ int compile_groups(dcs_plan *p, double *seconds) {
     if (seconds) *seconds = 0.0;
     if (p->executor != 0) return fail(DCS_ERR_INVALID, "a gather plan has no coset groups to compile");
     if (p->compiled) return DCS_OK;
-    CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0};
+    CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0, p->lds_buffers};
     std::string src = codegen::prelude(opt, kKernelsHpp);
     for (size_t gi = 0; gi < p->h_groups.size(); ++gi) {
         const GroupDev &g = p->h_groups[gi];
@@ -1184,11 +1284,14 @@ int compile_groups(dcs_plan *p, double *seconds) {
     }
     p->group_fn.assign(p->h_groups.size(), nullptr);
     p->group_grid.assign(p->h_groups.size(), 1);
+    p->group_threads.assign(p->h_groups.size(), kThreads);
     for (size_t gi = 0; gi < p->h_groups.size(); ++gi) {
         const std::string name = "dcs_group_" + std::to_string(gi);
         HIPCHK(hipModuleGetFunction(&p->group_fn[gi], mod, name.c_str()));
+        p->group_threads[gi] = codegen::group_threads(p->h_groups[gi]);
         int nb = 0;
-        HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, p->group_fn[gi], kThreads, 0));
+        HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, p->group_fn[gi], p->group_threads[gi], 0));
+        if (nb < 1) return fail(DCS_ERR_RUNTIME, "group kernel %zu cannot be resident (registers or LDS)", gi);
         const u64 cap = (u64)p->ctx->cus * (u64)std::max(1, nb);
         p->group_grid[gi] = (unsigned)std::max<u64>(1, std::min<u64>(p->h_groups[gi].ntiles, cap));
     }
@@ -1205,7 +1308,8 @@ int launch_compiled(dcs_plan *p, int nl, const Seg &sg, int nunits) {
     int nsteps = p->nsteps, nu = nunits, unit = sg.unit;
     u64 *dmax = p->d_dmax;
     void *args[] = {&E, &ls, &lanes, &dyn, &nsteps, &dmax, &nu, &unit};
-    PROF(2, HIPCHK(hipModuleLaunchKernel(p->group_fn[sg.group], p->group_grid[sg.group], (unsigned)nl, 1, kThreads, 1, 1, 0,
+    PROF(2, HIPCHK(hipModuleLaunchKernel(p->group_fn[sg.group], p->group_grid[sg.group], (unsigned)nl, 1,
+                                         (unsigned)p->group_threads[sg.group], 1, 1, 0,
                                          p->ctx->stream, args, nullptr)));
     return DCS_OK;
 }
@@ -1368,7 +1472,10 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     if (d->lanes < 1 || d->nsteps < 1 || d->blocks < 1) return fail(DCS_ERR_INVALID, "lanes, nsteps, blocks must be >= 1");
     if (d->k < 0 || d->k > 30 || popc(d->terminal_mask) > d->k)
         return fail(DCS_ERR_INVALID, "terminal mask holds more than k slots (k=%d)", d->k);
-    if (d->tile_log2 < 11 || d->tile_log2 > 12) return fail(DCS_ERR_INVALID, "tile_log2 must be 11 or 12");
+    if (d->tile_log2 < 11 || d->tile_log2 > 13)
+        return fail(DCS_ERR_INVALID, "tile_log2 must be 11 or 12 (13: DCS_MODE_COMPILED only)");
+    if (d->group_policy < 0 || d->group_policy > 1) return fail(DCS_ERR_INVALID, "group_policy must be 0 or 1");
+    if (d->lds_buffers < 0 || d->lds_buffers > 2) return fail(DCS_ERR_INVALID, "lds_buffers must be 0, 1 or 2");
     if (d->resident_bits != 0 && (d->resident_bits < 6 || d->resident_bits > 12))
         return fail(DCS_ERR_INVALID, "resident_bits must be 0 or 6..12");
     if (d->fuse_max_dim < 0 || d->fuse_max_dim > kMaxDimCompiled)
@@ -1392,6 +1499,8 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     p->exact_rounding = d->exact_rounding;
     p->pass_max_dim = d->pass_max_dim;
     p->lds_storage = d->lds_storage ? 1 : 0;
+    p->group_policy = d->group_policy;
+    p->lds_buffers = d->lds_buffers ? d->lds_buffers : 2;
     p->esz = d->storage == DCS_STORAGE_C64 ? sizeof(float2) : sizeof(double2);
     p->lane_stride = 1ull << d->maxslots;
     p->terminal_mask = d->terminal_mask;

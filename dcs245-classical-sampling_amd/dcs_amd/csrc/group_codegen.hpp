@@ -9,11 +9,15 @@
 // per-step formulas are those of dcs_kernels.hpp (orb_*), with the same operand order, so a step's
 // result is bit-identical to the per-step kernel's before rounding to storage.
 //
-// Pipeline of one workgroup (persistent over the tiles of its group):
+// Pipeline of one workgroup (persistent over the tiles of its group), two LDS tiles:
 //   store the previous tile's results (LDS -> HBM), issue the next tile's loads (HBM -> registers),
 //   run the register passes on the current tile (LDS), then move the loaded tile into the other LDS
 //   buffer.  Stores and loads are in flight during the passes; the only wait on global memory comes
 //   after them.
+// One LDS tile (half the LDS, twice the workgroups per CU): issue the next tile's loads, run the
+//   passes, then each thread stores its finished entries and puts the loaded ones in their place
+//   (the same LDS addresses, so no barrier is needed between the two); the stores drain during the
+//   next tile's passes.
 //
 // Host-side code (included by dcs_amd.hip after the descriptor types).
 #pragma once
@@ -28,6 +32,7 @@ struct CodegenOptions {
     int tile_log2;     // log2 of the largest tile (entries)
     bool exact;        // round to storage precision after every step
     bool lds_storage;  // LDS tiles in storage precision (rounded once per register pass) instead of FP64
+    int lds_buffers;   // LDS tiles per workgroup: 2 (double-buffered) or 1
 };
 
 namespace codegen {
@@ -287,20 +292,29 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
     emitf(o, "    }\n");
 }
 
+// Workgroup size of a group's kernel: one thread per 8 tile entries (256 .. 1024 threads), so that
+// larger tiles bring more waves rather than more registers per thread.
+inline int group_threads(const GroupDev &g) {
+    const uint32_t N = (1u << g.d) << g.wbits;
+    int t = 256;
+    while (t < 1024 && (uint32_t)t * 8 < N) t *= 2;
+    return t;
+}
+
 // Kernel `dcs_group_<gi>` of one fused group.
 //
-// Load / store mapping: thread tid owns elements e = tid + 256 k of every tile (element e = t * W + l
+// Load / store mapping: thread tid owns elements e = tid + T k of every tile, T threads (element e = t * W + l
 // sits at LDS index e).  Deposit and addrv are linear, so an element's address splits into a
 // per-thread part (pth, computed once) and a per-k constant.  (Pairing adjacent lanes into 16-byte
 // accesses measured slower on the 64-qubit plan and is not used.)
 inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupView &v) {
     const GroupDev &g = *v.g;
-    const int threads = 256;
+    const int threads = group_threads(g);
     const int D = g.d, WB = g.wbits;
     const uint32_t W = 1u << WB, N = (1u << D) << WB;
     const bool c64 = opt.storage == 0;
     const bool lds_is_storage = opt.lds_storage || !c64;  // Lt == St
-    const int EB = 8;  // bits of the per-thread element offset tid
+    const int EB = __builtin_ctz((unsigned)threads);  // bits of the per-thread element offset tid
     const uint32_t span = (uint32_t)threads;
     const int PER = N >= span ? (int)(N / span) : 1;
     const bool guard = N < (uint32_t)threads;
@@ -308,7 +322,8 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
     emitf(o, "extern \"C\" __global__ void __launch_bounds__(%d) dcs_group_%d(St *__restrict__ base, u64 lane_stride,\n", threads, gi);
     o += "    const int *__restrict__ lane_ids, const Dyn *__restrict__ dyn, int nsteps, u64 *__restrict__ dmax, int nunits,\n"
          "    int unit) {\n";
-    emitf(o, "  __shared__ Lt buf[%u];\n", 2 * N);
+    const bool single = opt.lds_buffers == 1;
+    emitf(o, "  __shared__ Lt buf[%u];\n", single ? N : 2 * N);
     emitf(o, "  const u64 NT = %lluull;\n", (unsigned long long)g.ntiles);
     o += "  u64 ti = blockIdx.x;\n  if (ti >= NT) return;\n"
          "  const uint32_t tid = threadIdx.x;\n"
@@ -400,9 +415,8 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         }
     }
     o += "  };\n";
-    o += "  auto store = [&](const Lt *P, u64 tbx) {\n";
-    for (int k = 0; k < PER; ++k) {
-        if (dout_k[k]) continue;
+    auto store_one = [&](int k) {
+        if (dout_k[k]) return;
         const uint32_t off = (uint32_t)k * span;
         const unsigned long long C = (unsigned long long)Ck[k];
         if (c64 && lds_is_storage) {
@@ -417,26 +431,57 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
             emitf(o, "    if (!tdout%s) put_scaled(E, tbx ^ pth ^ 0x%llxull, LG(P[%s]), scale, lm);\n", gexpr.c_str(), C,
                   guard ? "tid" : (std::string("vt + ") + std::to_string(off)).c_str());
         }
-    }
+    };
+    o += "  auto store = [&](Lt *P, u64 tbx) {\n";
+    for (int k = 0; k < PER; ++k) store_one(k);
     o += "  };\n";
+    if (single) {
+        // each thread reads its finished entry, stores it, then writes the loaded entry at the same
+        // LDS address: per-thread program order keeps the read before the write
+        o += "  auto store_put = [&](Lt *P, u64 tbx) {\n";
+        for (int k = 0; k < PER; ++k) {
+            store_one(k);
+            const uint32_t off = (uint32_t)k * span;
+            if (guard) emitf(o, "    if (tid < %uu) P[tid] = LP(Store<St>::load(R[%d]));\n", N, k);
+            else emitf(o, "    P[vt + %u] = LP(Store<St>::load(R[%d]));\n", off, k);
+        }
+        o += "  };\n";
+    }
     // pipeline
     emitf(o, "  u64 tb = tile_base(ti);\n  issue(tb);\n  put(buf);\n  __syncthreads();\n");
-    emitf(o, "  int c = 0;\n  u64 tprev = 0;\n  bool have_prev = false;\n");
-    emitf(o, "  for (;;) {\n");
-    emitf(o, "    Lt *T = buf + c * %u;\n    Lt *U = buf + (c ^ 1) * %u;\n", N, N);
-    o += "    if (have_prev) store(U, tprev);\n"
-         "    const u64 tn = ti + gridDim.x;\n"
-         "    const bool more = tn < NT;\n"
-         "    if (more) issue(tile_base(tn));\n";
-    for (int pi = 0; pi < g.npasses; ++pi) {
-        emit_pass(o, opt, v, v.passes[pi], threads);
-        if (pi + 1 < g.npasses) o += "    __syncthreads();\n";
+    if (single) {
+        o += "  for (;;) {\n"
+             "    Lt *T = buf;\n"
+             "    const u64 tn = ti + gridDim.x;\n"
+             "    const bool more = tn < NT;\n"
+             "    if (more) issue(tile_base(tn));\n";
+        for (int pi = 0; pi < g.npasses; ++pi) {
+            emit_pass(o, opt, v, v.passes[pi], threads);
+            o += "    __syncthreads();\n";
+        }
+        o += "    if (!more) { store(T, tb); break; }\n"
+             "    store_put(T, tb);\n"
+             "    __syncthreads();\n"
+             "    ti = tn;\n    tb = tile_base(ti);\n"
+             "  }\n";
+    } else {
+        emitf(o, "  int c = 0;\n  u64 tprev = 0;\n  bool have_prev = false;\n");
+        emitf(o, "  for (;;) {\n");
+        emitf(o, "    Lt *T = buf + c * %u;\n    Lt *U = buf + (c ^ 1) * %u;\n", N, N);
+        o += "    if (have_prev) store(U, tprev);\n"
+             "    const u64 tn = ti + gridDim.x;\n"
+             "    const bool more = tn < NT;\n"
+             "    if (more) issue(tile_base(tn));\n";
+        for (int pi = 0; pi < g.npasses; ++pi) {
+            emit_pass(o, opt, v, v.passes[pi], threads);
+            if (pi + 1 < g.npasses) o += "    __syncthreads();\n";
+        }
+        o += "    if (more) put(U);\n"
+             "    __syncthreads();\n"
+             "    if (!more) { store(T, tb); break; }\n"
+             "    have_prev = true;\n    tprev = tb;\n    c ^= 1;\n    ti = tn;\n    tb = tile_base(ti);\n"
+             "  }\n";
     }
-    o += "    if (more) put(U);\n"
-         "    __syncthreads();\n"
-         "    if (!more) { store(T, tb); break; }\n"
-         "    have_prev = true;\n    tprev = tb;\n    c ^= 1;\n    ti = tn;\n    tb = tile_base(ti);\n"
-         "  }\n";
     if (c64 && lds_is_storage) o += "  commit_max((double)lmf, &dmax[(u64)li * nunits + unit]);\n";
     else o += "  commit_max(lm, &dmax[(u64)li * nunits + unit]);\n";
     o += "}\n";

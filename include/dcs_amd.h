@@ -95,13 +95,17 @@ typedef struct {
     /* schedule knobs (DCS_MODE_FUSED) */
     int32_t fuse_max_dim;       /* largest coset dimension of a fused group: 1..8 (DCS_MODE_FUSED), 1..12 */
     int32_t resident_bits;      /* steps whose live slots all sit below this bit run LDS-resident, <= 12 */
-    int32_t tile_log2;          /* log2 entries per coset tile, 11 or 12 */
+    int32_t tile_log2;          /* log2 entries per coset tile, 11 or 12 (13: DCS_MODE_COMPILED) */
     int32_t exact_rounding;     /* 1: fused steps round to storage precision after every step (stored
                                    mantissas equal the per-step schedule); 0: round once per register
                                    pass (more accurate, fewer conversions) */
     int32_t pass_max_dim;       /* span of the orbit masks of one register pass, 3 or 4 */
     int32_t lds_storage;        /* DCS_MODE_COMPILED: 1 holds coset tiles in LDS in storage precision
                                    (rounded once per register pass), 0 in FP64 */
+    int32_t group_policy;       /* 0: greedy maximal coset groups; 1: the group boundaries with the fewest
+                                   HBM bytes (dynamic programming over the step sequence) */
+    int32_t lds_buffers;        /* DCS_MODE_COMPILED: LDS tiles per workgroup, 2 (double-buffered, the
+                                   default when 0) or 1 (the next tile waits in registers) */
 } dcs_plan_desc;
 
 typedef struct {
