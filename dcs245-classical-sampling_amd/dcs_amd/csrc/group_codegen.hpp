@@ -319,7 +319,9 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
     const int PER = N >= span ? (int)(N / span) : 1;
     const bool guard = N < (uint32_t)threads;
     std::string o;
-    emitf(o, "extern \"C\" __global__ void __launch_bounds__(%d) dcs_group_%d(St *__restrict__ base, u64 lane_stride,\n", threads, gi);
+    // at least 4 waves per SIMD (<= 128 VGPRs): two workgroups of 512 threads per CU
+    emitf(o, "extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(4))) "
+             "dcs_group_%d(St *__restrict__ base, u64 lane_stride,\n", threads, gi);
     o += "    const int *__restrict__ lane_ids, const Dyn *__restrict__ dyn, int nsteps, u64 *__restrict__ dmax, int nunits,\n"
          "    int unit) {\n";
     const bool single = opt.lds_buffers == 1;
