@@ -13,7 +13,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / 'libdcs_amd.so'
 
 DCS_OK, DCS_ERR_INVALID, DCS_ERR_RUNTIME, DCS_ERR_NOMEM, DCS_ERR_NONFINITE, DCS_ERR_ZERO = range(6)
-STORAGE_C64, STORAGE_C128 = 0, 1
+STORAGE_C64, STORAGE_C128, STORAGE_F16X2 = 0, 1, 2
 MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED, MODE_COMPILED = 0, 1, 2, 3
 
 EXPORTED = (
@@ -320,6 +320,11 @@ class Plan:
                 'cdf_lower_numpy': r.cdf_lower, 'cdf_upper_numpy': r.cdf_upper, 'cdf_margin': r.margin}
 
     def frontier_terminal(self, lane=0):
+        if self.storage == STORAGE_F16X2:
+            raw = np.empty((1 << self.terminal_rank, 2), np.uint16)
+            check(self.lib.dcs_frontier_read(self.handle, int(lane), raw.ctypes.data_as(C.c_void_p)))
+            f = raw.view(np.float16).astype(np.float32)
+            return (f[:, 0] + 1j * f[:, 1]).astype(np.complex64)
         dt = np.complex64 if self.storage == STORAGE_C64 else np.complex128
         out = np.empty(1 << self.terminal_rank, dt)
         check(self.lib.dcs_frontier_read(self.handle, int(lane), out.ctypes.data_as(C.c_void_p)))

@@ -243,6 +243,8 @@ class ConditionalBlockGPU:
 class InplaceConditionalBlockGPU(ConditionalBlockGPU):
     """Reference-compatible in-place engine (conditional_block_inplace_gpu.py:17-38, 194-215)."""
 
+    _c64_storage = _native.STORAGE_C64   # frontier storage behind dtype=np.complex64
+
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
               fuse_max_dim=8, resident_bits=12, tile_log2=12, blocks=1, exact_rounding=False,
               pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=1):
@@ -281,7 +283,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
         self.mode = SCHEDULES[schedule]
         self.lanes = int(lanes)
         self.ctx = _native.Context(device)
-        storage = _native.STORAGE_C128 if self.use_fp64 else _native.STORAGE_C64
+        storage = _native.STORAGE_C128 if self.use_fp64 else self._c64_storage
         self._plan = _native.Plan(self.ctx, bp.step_table(), bp.inplace_maxslots, storage, self.lanes, bp.k,
                                   bp.terminal_mask(), bp.syndrome_map(), bp.phase(), bp.open_unitaries(),
                                   blocks=blocks, fuse_max_dim=fuse_max_dim, resident_bits=resident_bits,
@@ -313,3 +315,27 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
     def _sweep(self, f, yfix, renorm_every=None):
         env, ls, _ = self.prefix_inplace(f, yfix)
         return (0j if env is None else complex(env[0])), ls
+
+
+class HalfConditionalBlockGPU(InplaceConditionalBlockGPU):
+    """Binary16 storage of the in-place frontier, FP64 arithmetic (sampling/conditional_block_half_gpu.py:8-34,
+    kernel_opt/inplace_half_gpu.py:14-78): each entry is two independently rounded IEEE float16 parts
+    (4 bytes), stored through float with round to nearest; maxima and the scale chain use the stored
+    values; the coherent slice sum stays complex128.  An approximate variant: whole-vector quality needs
+    an FP64 check (the reference's own scope note)."""
+
+    _c64_storage = _native.STORAGE_F16X2
+
+    def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, **kw):
+        if np.dtype(dtype) != np.dtype(np.complex64):
+            raise ValueError('Half storage exports complex64; use baseline executor for complex128')
+        super().alloc(dtype=np.complex64, device=device, fp64_kernel_path=fp64_kernel_path, **kw)
+        self.inplace_storage = 'float16-components'
+
+    def block(self, *args, **kwargs):
+        vector, meta = super().block(*args, **kwargs)
+        meta.update({'intermediate_storage': 'float16 real/imaginary components',
+                     'local_arithmetic': 'FP64 then float32-to-float16 rounded storage',
+                     'coherent_accumulation': 'complex128',
+                     'approximation_scope': 'Whole-vector quality requires independent FP64 validation'})
+        return vector, meta

@@ -443,11 +443,7 @@ __global__ void k_fill_u64(u64 *p, u64 n, u64 v) {
 template <class S>
 __global__ void k_init_lanes(S *base, u64 lane_stride, const int *lane_ids, int nlanes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nlanes) {
-        S one = Store<S>::zero();
-        one.x = 1;
-        base[(u64)lane_ids[i] * lane_stride] = one;
-    }
+    if (i < nlanes) base[(u64)lane_ids[i] * lane_stride] = Store<S>::round(cd{1.0, 0.0});
 }
 
 template <class S>
@@ -753,6 +749,10 @@ int ev_resolve(dcs_plan *p) {
         rc_ = ev_end(p, pe_);                          \
         if (rc_) return rc_;                           \
     } while (0)
+
+size_t storage_bytes(int storage) {
+    return storage == DCS_STORAGE_C64 ? sizeof(float2) : storage == DCS_STORAGE_C128 ? sizeof(double2) : sizeof(h16x2);
+}
 
 unsigned grid_for(u64 work, int cus, unsigned per_cu) {
     const u64 blocks = (work + kThreads - 1) / kThreads;
@@ -1466,8 +1466,8 @@ void dcs_plan_destroy(dcs_plan *p) {
 int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     if (!ctx || !d || !out) return fail(DCS_ERR_INVALID, "null argument");
     *out = nullptr;
-    if (d->storage != DCS_STORAGE_C64 && d->storage != DCS_STORAGE_C128)
-        return fail(DCS_ERR_INVALID, "storage must be 0 (complex64) or 1 (complex128)");
+    if (d->storage != DCS_STORAGE_C64 && d->storage != DCS_STORAGE_C128 && d->storage != DCS_STORAGE_F16X2)
+        return fail(DCS_ERR_INVALID, "storage must be 0 (complex64), 1 (complex128) or 2 (binary16 pairs)");
     if (d->maxslots < 0 || d->maxslots > 40) return fail(DCS_ERR_INVALID, "maxslots %d out of range", d->maxslots);
     if (d->lanes < 1 || d->nsteps < 1 || d->blocks < 1) return fail(DCS_ERR_INVALID, "lanes, nsteps, blocks must be >= 1");
     if (d->k < 0 || d->k > 30 || popc(d->terminal_mask) > d->k)
@@ -1501,7 +1501,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     p->lds_storage = d->lds_storage ? 1 : 0;
     p->group_policy = d->group_policy;
     p->lds_buffers = d->lds_buffers ? d->lds_buffers : 2;
-    p->esz = d->storage == DCS_STORAGE_C64 ? sizeof(float2) : sizeof(double2);
+    p->esz = storage_bytes(d->storage);
     p->lane_stride = 1ull << d->maxslots;
     p->terminal_mask = d->terminal_mask;
     p->unitaries.assign(d->open_unitaries, d->open_unitaries + (size_t)d->k * 8);
@@ -1681,7 +1681,7 @@ int dcs_gather_plan_create(dcs_context *ctx, const dcs_gather_desc *d, dcs_plan 
     p->nsteps = d->nsteps;
     p->k = d->k;
     p->blocks = d->blocks;
-    p->esz = d->storage == DCS_STORAGE_C64 ? sizeof(float2) : sizeof(double2);
+    p->esz = storage_bytes(d->storage);
     p->lane_stride = 1ull << d->rmax;
     p->terminal_mask = (d->terminal_rank == 0) ? 0ull : ((1ull << d->terminal_rank) - 1);
     p->unitaries.assign(d->open_unitaries, d->open_unitaries + (size_t)d->k * 8);
@@ -1822,9 +1822,12 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
     int rc;
     if (p->executor == 1)
         rc = (p->storage == DCS_STORAGE_C64) ? run_gather<float2>(p, mode) : run_gather<double2>(p, mode);
+    else if (p->storage == DCS_STORAGE_C64)
+        rc = run_contract<float2>(p, mode, nl, p->group_ntiles);
+    else if (p->storage == DCS_STORAGE_C128)
+        rc = run_contract<double2>(p, mode, nl, p->group_ntiles);
     else
-        rc = (p->storage == DCS_STORAGE_C64) ? run_contract<float2>(p, mode, nl, p->group_ntiles)
-                                             : run_contract<double2>(p, mode, nl, p->group_ntiles);
+        rc = run_contract<h16x2>(p, mode, nl, p->group_ntiles);
     if (rc) {
         (void)hipStreamSynchronize(st);
         return rc;
@@ -1857,6 +1860,9 @@ int dcs_block_accumulate(dcs_plan *p, int block, int lane, uint64_t xmask, doubl
     if (p->storage == DCS_STORAGE_C64) {
         const float2 *E = (const float2 *)p->frontier + (u64)lane * p->lane_stride;
         PROF(3, (k_scatter<float2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, p->d_smap, xmask, cd{cr, ci}, tot)));
+    } else if (p->storage == DCS_STORAGE_F16X2) {
+        const h16x2 *E = (const h16x2 *)p->frontier + (u64)lane * p->lane_stride;
+        PROF(3, (k_scatter<h16x2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, p->d_smap, xmask, cd{cr, ci}, tot)));
     } else {
         const double2 *E = (const double2 *)p->frontier + (u64)lane * p->lane_stride;
         PROF(3, (k_scatter<double2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, p->d_smap, xmask, cd{cr, ci}, tot)));
@@ -1931,6 +1937,9 @@ int dcs_frontier_read(dcs_plan *p, int lane, void *host_out) {
     if (p->storage == DCS_STORAGE_C64) {
         const float2 *E = (const float2 *)p->frontier + (u64)lane * p->lane_stride;
         k_gather_terminal<float2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, (float2 *)p->d_scratch);
+    } else if (p->storage == DCS_STORAGE_F16X2) {
+        const h16x2 *E = (const h16x2 *)p->frontier + (u64)lane * p->lane_stride;
+        k_gather_terminal<h16x2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, (h16x2 *)p->d_scratch);
     } else {
         const double2 *E = (const double2 *)p->frontier + (u64)lane * p->lane_stride;
         k_gather_terminal<double2><<<g, kThreads, 0, p->ctx->stream>>>(E, term, n, (double2 *)p->d_scratch);

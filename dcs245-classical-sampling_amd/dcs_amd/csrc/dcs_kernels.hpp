@@ -41,6 +41,35 @@ struct Store<float2> {
     __device__ static __forceinline__ float2 zero() { return make_float2(0.f, 0.f); }
 };
 
+// IEEE binary16 storage of the real and imaginary parts (4 bytes per entry), the reference's
+// half-storage engine (kernel_opt/inplace_half_gpu.py:14-28): a value is loaded as half -> float ->
+// double and stored as double -> float (round to nearest) -> half (round to nearest even); |stored|^2
+// is formed in float from the stored halves.
+struct __align__(4) h16x2 {
+    unsigned short x, y;
+};
+
+__device__ __forceinline__ float h16_to_f32(unsigned short h) { return (float)__builtin_bit_cast(_Float16, h); }
+__device__ __forceinline__ unsigned short f32_to_h16(float f) {
+    return __builtin_bit_cast(unsigned short, (_Float16)f);
+}
+
+template <>
+struct Store<h16x2> {
+    __device__ static __forceinline__ cd load(h16x2 v) { return cd{(double)h16_to_f32(v.x), (double)h16_to_f32(v.y)}; }
+    __device__ static __forceinline__ h16x2 round(cd v) {
+        h16x2 o;
+        o.x = f32_to_h16((float)v.r);
+        o.y = f32_to_h16((float)v.i);
+        return o;
+    }
+    __device__ static __forceinline__ double norm2(h16x2 v) {
+        const float r = h16_to_f32(v.x), i = h16_to_f32(v.y);
+        return (double)(r * r + i * i);
+    }
+    __device__ static __forceinline__ h16x2 zero() { return h16x2{0, 0}; }
+};
+
 template <>
 struct Store<double2> {
     __device__ static __forceinline__ cd load(double2 v) { return cd{v.x, v.y}; }

@@ -28,7 +28,7 @@
 #include <vector>
 
 struct CodegenOptions {
-    int storage;       // DCS_STORAGE_C64 / DCS_STORAGE_C128
+    int storage;       // DCS_STORAGE_C64 / DCS_STORAGE_C128 / DCS_STORAGE_F16X2
     int tile_log2;     // log2 of the largest tile (entries)
     bool exact;        // round to storage precision after every step
     bool lds_storage;  // LDS tiles in storage precision (rounded once per register pass) instead of FP64
@@ -65,8 +65,11 @@ inline std::string prelude(const CodegenOptions &opt, const char *kernels_hpp) {
          "typedef unsigned long long uint64_t;\ntypedef long long int64_t;\n";
     o += kernels_hpp;
     o += "\n";
-    const char *st = opt.storage == 0 ? "float2" : "double2";
-    const char *lt = (opt.lds_storage || opt.storage == 1) ? st : "double2";
+    // LDS tiles: complex64 storage with lds_storage holds tiles as complex64, complex128 as itself;
+    // binary16 storage holds them as complex64 (lds_storage) or FP64, never as binary16
+    const char *st = opt.storage == 0 ? "float2" : opt.storage == 1 ? "double2" : "h16x2";
+    const char *lt = opt.storage == 2 ? (opt.lds_storage ? "float2" : "double2")
+                                      : ((opt.lds_storage || opt.storage == 1) ? st : "double2");
     emitf(o, "typedef %s St;\ntypedef %s Lt;\n", st, lt);
     o += R"(
 __device__ __forceinline__ cd cneg(cd a) { return cd{-a.r, -a.i}; }
@@ -176,9 +179,9 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
             if (q.kind == 4) emitf(o, "      const cd C%d_0 = q.y ? F%d : f0, C%d_1 = q.y ? cneg(F%d) : f0;\n", m, m, m, m);
         }
         // Elementary updates of this step, each keyed by the element playing "c" (an update is needed
-        // only when that element is live).  With complex64 storage outside the per-step rounding mode,
-        // two-product sums use one chain of fused multiply-adds.
-        const bool fma2 = opt.storage == 0 && !opt.exact;
+        // only when that element is live).  With complex64 or binary16 storage outside the per-step
+        // rounding mode, two-product sums use one chain of fused multiply-adds.
+        const bool fma2 = opt.storage != 1 && !opt.exact;
         for (int m = 0; m < OPT; ++m) {
             std::vector<std::pair<int, std::string>> ops;
             char buf[512];
@@ -313,7 +316,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
     const int D = g.d, WB = g.wbits;
     const uint32_t W = 1u << WB, N = (1u << D) << WB;
     const bool c64 = opt.storage == 0;
-    const bool lds_is_storage = opt.lds_storage || !c64;  // Lt == St
+    const bool lds_is_storage = opt.lds_storage || opt.storage == 1;  // Lt == St (complex64 / complex128)
     const int EB = __builtin_ctz((unsigned)threads);  // bits of the per-thread element offset tid
     const uint32_t span = (uint32_t)threads;
     const int PER = N >= span ? (int)(N / span) : 1;

@@ -59,8 +59,10 @@ typedef struct {
     int32_t device;
 } dcs_device_info;
 
-/* Frontier storage of the contraction.  Arithmetic is FP64 in both cases. */
-enum { DCS_STORAGE_C64 = 0, DCS_STORAGE_C128 = 1 };
+/* Frontier storage of the contraction.  Arithmetic is FP64 in every case.  DCS_STORAGE_F16X2 holds
+ * the real and imaginary parts as IEEE binary16 (4 bytes per entry), stored through float with round
+ * to nearest, the half-storage engine of kernel_opt/inplace_half_gpu.py:14-28 (in-place plans only). */
+enum { DCS_STORAGE_C64 = 0, DCS_STORAGE_C128 = 1, DCS_STORAGE_F16X2 = 2 };
 
 /* Execution schedules of dcs_contract. */
 enum {

@@ -6,7 +6,10 @@
  *   oracle_contract        kernel_opt/inplace_plan.py:90-135 (execute_inplace, the eight orbit
  *                          updates) with the scale chain of the GPU driver
  *                          kernel_opt/inplace_gpu.py:242-258 (step j multiplied by 2^-e,
- *                          e = frexp(sqrt(max |out_{j-1}|^2)).exp; max of the stored values)
+ *                          e = frexp(sqrt(max |out_{j-1}|^2)).exp; max of the stored values); storage
+ *                          complex64, complex128 or binary16 pairs (kernel_opt/inplace_half_gpu.py:14-78)
+ *   oracle_f32_to_h16,     the binary16 conversions of that storage
+ *   oracle_h16_to_f32
  *   oracle_gather          dcs_rankwidth/scan_plan.py:161-195 and dcs_rankwidth/gpu_kernel.py:58-101
  *                          (the canonical gather executor: out[c'] = sum over kernel combinations of
  *                          E[p & mask] * (+-f_y), p = XOR of byte tables ^ combination) with the
@@ -43,26 +46,81 @@ static inline u64 pdep(u64 x, u64 mask) {
     return r;
 }
 
-/* storage access: storage 0 = complex64 (float pairs), 1 = complex128 */
+/* IEEE binary16 <-> binary32, bit for bit (the conversions of kernel_opt/inplace_half_gpu.py:14-21:
+ * half -> float is exact; float -> half rounds to nearest, ties to even, with subnormal halves and
+ * overflow to infinity). */
+static inline float h16_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    uint32_t x;
+    if (e == 0) {
+        if (m == 0) x = sign;
+        else {
+            const float f = (float)m * 5.9604644775390625e-08f; /* m * 2^-24, exact */
+            memcpy(&x, &f, 4);
+            x |= sign;
+        }
+    } else if (e == 31) x = sign | 0x7f800000u | (m << 13);
+    else x = sign | ((e + 112u) << 23) | (m << 13);
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+static inline uint16_t f32_to_h16(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint16_t sign = (uint16_t)((x >> 16) & 0x8000u);
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? (0x200u | ((ax >> 13) & 0x3ffu)) : 0u));
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* >= 65520 rounds to infinity */
+    if (ax < 0x38800000u) {                                   /* below 2^-14: subnormal half or zero */
+        if (ax < 0x33000000u) return sign;                    /* below 2^-25 (2^-25 itself ties to 0) */
+        const uint32_t e = ax >> 23, mant = (ax & 0x7fffffu) | 0x800000u;
+        const uint32_t shift = 126u - e; /* value = mant * 2^(e-150) = h * 2^-24 */
+        uint32_t h = mant >> shift;
+        const uint32_t rem = mant & ((1u << shift) - 1u), halfway = 1u << (shift - 1u);
+        if (rem > halfway || (rem == halfway && (h & 1u))) ++h;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t r = ax - 0x38000000u; /* exponent rebias 127 -> 15 */
+    r += 0xfffu + ((r >> 13) & 1u);
+    return (uint16_t)(sign | (r >> 13));
+}
+
+/* storage access: storage 0 = complex64 (float pairs), 1 = complex128, 2 = binary16 pairs */
 static inline cd ld(const void *E, int st, u64 c) {
     cd o;
-    if (st) { const double *p = (const double *)E + 2 * c; o.r = p[0]; o.i = p[1]; }
-    else { const float *p = (const float *)E + 2 * c; o.r = p[0]; o.i = p[1]; }
+    if (st == 1) { const double *p = (const double *)E + 2 * c; o.r = p[0]; o.i = p[1]; }
+    else if (st == 0) { const float *p = (const float *)E + 2 * c; o.r = p[0]; o.i = p[1]; }
+    else { const uint16_t *p = (const uint16_t *)E + 2 * c; o.r = h16_to_f32(p[0]); o.i = h16_to_f32(p[1]); }
     return o;
 }
-/* store v * scale rounded to storage; returns |stored|^2 in storage precision */
+/* store v * scale rounded to storage; returns |stored|^2 in storage precision (float for binary16,
+ * inplace_half_gpu.py:23-26) */
 static inline double put(void *E, int st, u64 c, cd v, double scale) {
-    if (st) {
+    if (st == 1) {
         double *p = (double *)E + 2 * c;
         double r = v.r * scale, i = v.i * scale;
         p[0] = r; p[1] = i;
         return r * r + i * i;
-    } else {
+    } else if (st == 0) {
         float *p = (float *)E + 2 * c;
         float r = (float)(v.r * scale), i = (float)(v.i * scale);
         p[0] = r; p[1] = i;
         return (double)(r * r + i * i);
+    } else {
+        uint16_t *p = (uint16_t *)E + 2 * c;
+        p[0] = f32_to_h16((float)(v.r * scale));
+        p[1] = f32_to_h16((float)(v.i * scale));
+        const float r = h16_to_f32(p[0]), i = h16_to_f32(p[1]);
+        return (double)(r * r + i * i);
     }
+}
+
+void oracle_f32_to_h16(const float *in, uint16_t *out, u64 n) {
+    for (u64 j = 0; j < n; ++j) out[j] = f32_to_h16(in[j]);
+}
+void oracle_h16_to_f32(const uint16_t *in, float *out, u64 n) {
+    for (u64 j = 0; j < n; ++j) out[j] = h16_to_f32(in[j]);
 }
 
 static inline double maxnan(double a, double b) {
@@ -113,8 +171,9 @@ static u64 orbit_slots(int kind, u64 in_mask, int pivot, int pivot_a) {
 int oracle_contract(int storage, int nsteps, const int32_t *kind, const u64 *K, const u64 *Ka,
                     const int32_t *pivot, const int32_t *pivot_a, const u64 *beta, const u64 *in_mask,
                     const double *coeffs, const int32_t *ov, const int32_t *y, void *E, double *unit_m2) {
-    if (storage) { ((double *)E)[0] = 1.0; ((double *)E)[1] = 0.0; }
-    else { ((float *)E)[0] = 1.0f; ((float *)E)[1] = 0.0f; }
+    if (storage == 1) { ((double *)E)[0] = 1.0; ((double *)E)[1] = 0.0; }
+    else if (storage == 0) { ((float *)E)[0] = 1.0f; ((float *)E)[1] = 0.0f; }
+    else { ((uint16_t *)E)[0] = 0x3c00u; ((uint16_t *)E)[1] = 0u; }
     double prev = 1.0;
     for (int s = 0; s < nsteps; ++s) {
         const int kd = kind[s];
@@ -214,6 +273,7 @@ int oracle_gather(int storage, int nsteps, const int32_t *r_in, const int32_t *r
                   const int32_t *ov, const int32_t *y, void *A, void *B, double *unit_m2) {
     void *buf[2] = {A, B};
     int cur = 0;
+    if (storage != 0 && storage != 1) return -1; /* the canonical executor has no binary16 storage */
     if (storage) { ((double *)A)[0] = 1.0; ((double *)A)[1] = 0.0; }
     else { ((float *)A)[0] = 1.0f; ((float *)A)[1] = 0.0f; }
     double prev = 1.0;

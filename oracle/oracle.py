@@ -6,7 +6,10 @@ as the checker (or, for cpu_baseline, as the timed CPU port).  The product path 
 What each function restates and what pins it:
 
   contract_inplace   C restatement (liboracle.so) of execute_inplace (kernel_opt/inplace_plan.py:90-135)
-                     with the GPU driver's scale chain (kernel_opt/inplace_gpu.py:242-258).
+                     with the GPU driver's scale chain (kernel_opt/inplace_gpu.py:242-258); storage 2 is
+                     the binary16-pair storage of kernel_opt/inplace_half_gpu.py:14-78, whose
+                     conversions (f32_to_h16, h16_to_f32) are pinned against numpy's float16 over every
+                     half and over float32 boundary and random patterns.
                      Pinned by: the dense state vector of the Clifford+T circuit (statevector() below,
                      closed-form gate matrices) through block(); by gather_amplitude(), an independent
                      restatement of the other reference executor (dcs_rankwidth/scan_plan.py:161-195)
@@ -63,6 +66,10 @@ def lib():
         L.oracle_draw.restype = C.c_int
         L.oracle_draw.argtypes = [dp, C.c_uint64, C.c_double, dp, dp]
         L.oracle_threads.restype = C.c_int
+        L.oracle_f32_to_h16.restype = None
+        L.oracle_f32_to_h16.argtypes = [P(C.c_float), P(C.c_uint16), C.c_uint64]
+        L.oracle_h16_to_f32.restype = None
+        L.oracle_h16_to_f32.argtypes = [P(C.c_uint16), P(C.c_float), C.c_uint64]
         _lib = L
     return _lib
 
@@ -84,13 +91,47 @@ def frexp_scale_chain(m2):
     return ls, True
 
 
+STORAGE_DTYPES = {0: np.complex64, 1: np.complex128, 2: np.dtype((np.uint16, 2))}
+
+
+def frontier_buffer(storage, width):
+    """Zeroed frontier of 2^width entries: complex64 (storage 0), complex128 (1) or binary16 pairs (2,
+    uint16[2^width, 2])."""
+    if storage == 2:
+        return np.zeros((1 << width, 2), np.uint16)
+    return np.zeros(1 << width, STORAGE_DTYPES[storage])
+
+
+def f32_to_h16(x):
+    """float32 -> binary16 bits (round to nearest even), as the oracle stores binary16 entries."""
+    a = np.ascontiguousarray(x, np.float32)
+    out = np.empty(a.shape, np.uint16)
+    lib().oracle_f32_to_h16(_p(a, C.c_float), _p(out, C.c_uint16), a.size)
+    return out
+
+
+def h16_to_f32(h):
+    a = np.ascontiguousarray(h, np.uint16)
+    out = np.empty(a.shape, np.float32)
+    lib().oracle_h16_to_f32(_p(a, C.c_uint16), _p(out, C.c_float), a.size)
+    return out
+
+
+def as_complex(E, storage):
+    """Complex view of a frontier in storage precision (binary16 pairs widened to complex64)."""
+    if storage != 2:
+        return E
+    f = h16_to_f32(E)
+    return (f[..., 0] + 1j * f[..., 1]).astype(np.complex64)
+
+
 def contract_inplace(bp, f, yfix, storage=1, buffer=None):
-    """Run the in-place plan of BlockPlan `bp` for one slice.  Returns dict(E, m2, ls, alive, offset)."""
+    """Run the in-place plan of BlockPlan `bp` for one slice.  Returns dict(E, m2, ls, alive, offset).
+    storage: 0 complex64, 1 complex128, 2 binary16 pairs (kernel_opt/inplace_half_gpu.py)."""
     t = bp.step_table()
     coeffs, ov, ys, offset = bp.step_parameters(f, yfix)
     n = len(t['kind'])
-    dt = np.complex128 if storage else np.complex64
-    E = buffer if buffer is not None else np.zeros(1 << bp.inplace_maxslots, dt)
+    E = buffer if buffer is not None else frontier_buffer(storage, bp.inplace_maxslots)
     m2 = np.zeros(n, np.float64)
     arrs = {k: np.ascontiguousarray(t[k]) for k in ('kind', 'K', 'Ka', 'pivot', 'pivot_a', 'beta', 'in_mask')}
     lib().oracle_contract(int(storage), n, _p(arrs['kind'], C.c_int32), _p(arrs['K'], C.c_uint64),
@@ -117,8 +158,7 @@ def block(bp, x_bits, w=None, storage=1, part=0, parts=1):
     tv = total.view(np.float64)
     smap = np.ascontiguousarray(bp.syndrome_map())
     f_full = bp.local_vectors(x_bits, w=w)
-    dt = np.complex128 if storage else np.complex64
-    E = np.zeros(1 << bp.inplace_maxslots, dt)
+    E = frontier_buffer(storage, bp.inplace_maxslots)
     for _, coef, f, yfix, freeflip in bp.slice_assignments(f_full, part, parts):
         if f is None:
             continue

@@ -109,3 +109,38 @@ def test_ibm64_driver_worker_with_audit_matches_r1(ibm64, records, tmp_path):
     assert abs(w['loss'] - a['loss']) <= 2e-6
     hw = json.loads((out / 'hardware.json').read_text())
     assert hw['arch'].startswith('gfx950')
+
+
+def test_ibm64_half_k1_whole_vector_against_c128(ibm64, records):
+    """Binary16 storage on the frozen half_k1 plan (2 parity slices, rmax 35: a 2^35-entry, 128 GiB
+    frontier of binary16 pairs), compiled schedule, first protocol prefix: the whole 2^20 block against
+    the complex128 k3 block (the FP64 validation the reference's half engine asks for,
+    conditional_block_half_gpu.py:3-4 and 31-32): normalised overlap, block mass and the largest
+    deviation after aligning the global phase."""
+    from dcs_amd.conditional_block import HalfConditionalBlockGPU
+    s = records['samples']['0']
+    print('case ibm64 half_k1 compiled sample 0', flush=True)
+    e = HalfConditionalBlockGPU(ibm64['graph'], ibm64['unitaries'], 64, ibm64['order'],
+                                ibm64['plans']['half_k1']['operations'], ibm64['open_vertices'])
+    e.alloc()
+    assert e.inplace_rmax == ibm64['plans']['half_k1']['rmax'] == 35
+    vec, meta = e.block(bits(s['bits']))
+    half = np.asarray(vec)
+    e.close()
+    assert meta['slices_evaluated'] == 2 and meta['coherent_accumulation'] == 'complex128'
+    e = make(ibm64, 'inplace_k3_fp64', np.complex128)
+    ref = np.asarray(e.block(bits(s['bits']))[0])
+    e.close()
+    overlap = abs(np.vdot(ref, half)) / np.sqrt(np.vdot(ref, ref).real * np.vdot(half, half).real)
+    A = records['scores']['0'][0]
+    mass_h, mass_r = float(np.vdot(half, half).real), float(np.vdot(ref, ref).real)
+    print(f'half: {meta["seconds"]:.2f} s, overlap {overlap:.8f}, mass {mass_h:.6f} vs {mass_r:.6f}, '
+          f'|amp| {abs(half[A]):.6e} vs {abs(ref[A]):.6e}, max rel err {rel_err(half, ref):.3e}', flush=True)
+    # observed: overlap 0.99999813, masses 1.000012 / 0.999888, largest deviation 1.8e-3 of the block max
+    assert overlap > 0.9999
+    assert abs(mass_h - mass_r) < 1e-3
+    assert rel_err(half, ref) < 1e-2
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b * (np.vdot(b, a) / np.vdot(b, b)))) / np.max(np.abs(b)))

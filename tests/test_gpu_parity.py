@@ -311,3 +311,45 @@ def test_device_draw_random_vectors():
         assert d['A_index'] == idx and d['scaled_mass_cdf'] == tot
         assert d['cdf_upper_numpy'] == cum[idx] and d['cdf_lower_numpy'] == (cum[idx - 1] if idx else 0.)
     e.close()
+
+
+@pytest.mark.parametrize('sched', ['steps', 'steps_host'])
+def test_c1_half_storage_per_step_bit_exact(c1, sched):
+    """Binary16 storage (kernel_opt/inplace_half_gpu.py): the per-step schedules equal the oracle's
+    half-storage block bit for bit (same conversions, same products, same scale chain)."""
+    from dcs_amd.conditional_block import HalfConditionalBlockGPU
+    print(f'case c1 half {sched}', flush=True)
+    ref = oracle.block(BlockPlan(*c1['args']), c1['x'], storage=2)
+    e = HalfConditionalBlockGPU(*c1['args'])
+    e.alloc(schedule=sched)
+    v, meta = e.block(c1['x'])
+    assert np.array_equal(np.asarray(v), ref)
+    assert meta['intermediate_storage'] == 'float16 real/imaginary components'
+    assert e.inplace_storage == 'float16-components'
+    e.close()
+
+
+@pytest.mark.parametrize('sched', ['fused', 'compiled'])
+def test_w20_half_storage_fused(w20, sched):
+    """Binary16 storage on the width-20 plan: the per-step schedule equals the oracle bit for bit.  Fused
+    schedules apply the power-of-two scale once per group; unlike complex64, binary16 rounding does not
+    commute with that scale (its subnormal floor 2^-24 is absolute), so fused blocks are held to
+    accuracy instead: with per-step or per-pass rounding, no further from FP64 than the per-step
+    binary16 schedule (x2)."""
+    from dcs_amd.conditional_block import HalfConditionalBlockGPU
+    print(f'case w20 half {sched}', flush=True)
+    bp = BlockPlan(*w20['args'])
+    ref16 = oracle.block(bp, w20['x'], storage=2)
+    e = HalfConditionalBlockGPU(*w20['args'])
+    e.alloc(schedule='steps')
+    assert np.array_equal(np.asarray(e.block(w20['x'])[0]), ref16)
+    e.close()
+    err_steps = rel(ref16, w20['ref128'])
+    for exact in (True, False):
+        e = HalfConditionalBlockGPU(*w20['args'])
+        e.alloc(schedule=sched, exact_rounding=exact)
+        v = np.asarray(e.block(w20['x'])[0])
+        e.close()
+        err = rel(v, w20['ref128'])
+        print(f'half {sched} exact={exact} error {err:.3e}, per-step half error {err_steps:.3e}', flush=True)
+        assert err <= 2 * err_steps, (exact, err, err_steps)

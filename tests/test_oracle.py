@@ -198,3 +198,50 @@ def test_structurally_fixed_reference_rule():
     f = local_vectors(U, inst['N'], 8, np.zeros(8, np.uint8))
     for q in fx:
         assert min(abs(f[q][0]), abs(f[q][1])) < 1e-12
+
+
+def test_binary16_conversions_match_numpy():
+    """The oracle's binary16 conversions (the half-storage engine's cvt.f32.f16 / cvt.rn.f16.f32,
+    kernel_opt/inplace_half_gpu.py:14-21) against numpy's float16: every half exactly, and float32 ->
+    half on every tie and neighbour of every half interval plus a strided sweep of all float32 patterns
+    (normal, subnormal, overflow, infinities, NaN)."""
+    h = np.arange(1 << 16, dtype=np.uint32).astype(np.uint16)
+    got = oracle.h16_to_f32(h)
+    want = h.view(np.float16).astype(np.float32)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got.view(np.uint32)[~nan], want.view(np.uint32)[~nan])
+    # midpoints between consecutive finite halves (exact in float32) and one float32 ulp either side
+    fin = want[np.isfinite(want) & (want >= 0)]
+    fin = np.unique(fin)
+    mids = ((fin[:-1].astype(np.float64) + fin[1:]) / 2).astype(np.float32)
+    probes = np.concatenate([mids, np.nextafter(mids, np.float32(0)), np.nextafter(mids, np.float32(np.inf)),
+                             fin, np.float32([65504, 65519.996, 65520, 65536, 1e-8, 2.0 ** -25, 2.0 ** -24,
+                                              3 * 2.0 ** -26, 0.0, np.inf, np.nan])])
+    strided = (np.arange(0, 1 << 32, 65537, dtype=np.uint64)).astype(np.uint32).view(np.float32)
+    x = np.concatenate([probes, -probes, strided])
+    with np.errstate(over='ignore', invalid='ignore'):
+        want16 = x.astype(np.float16)
+    got16 = oracle.f32_to_h16(x)
+    wn = np.isnan(want16)
+    assert np.array_equal(np.isnan(got16.view(np.float16)), wn)
+    assert np.array_equal(got16[~wn], want16.view(np.uint16)[~wn])
+
+
+def test_binary16_storage_block_tracks_statevector_c1():
+    """Half-storage contraction of C1 (16q/d12/20T, all 2^16 amplitudes): within binary16 accuracy of
+    the dense state vector (rounding 2^-11 per stored part per step, about 20 steps deep), unit total
+    probability within the same accuracy, and the complex64 path strictly closer."""
+    inst = synthetic_instance(16, 12, 20, seed=0)
+    psi = oracle.statevector(inst['ops'], 16)
+    bp = BlockPlan(inst['graph'], inst['unitaries'], 16, inst['natural_order'], [], list(range(16)))
+    x = np.zeros(16, np.uint8)
+    errs = {}
+    for storage in (0, 2):
+        v = oracle.block(bp, x, storage=storage)
+        i = int(np.argmax(np.abs(psi)))
+        ph = v[i] / psi[i]
+        errs[storage] = np.max(np.abs(v - ph * psi)) / np.max(np.abs(psi))
+        assert abs(np.vdot(v, v).real - 1) < (1e-3 if storage == 2 else 1e-6)
+    assert errs[2] < 1e-3, errs          # observed 1.1e-4
+    assert errs[0] < 1e-6 and errs[0] < errs[2] / 100, errs
