@@ -79,10 +79,18 @@ class Server:
 
     # ---- admission ---------------------------------------------------------------------------
     def register(self, name, graph, unitaries, n, order, parity_operations, open_vertices, dtype=np.complex64,
-                 lanes=4, max_batch=4, **alloc_kw):
+                 lanes=4, max_batch=4, autotune=False, tune_cache=None, **alloc_kw):
+        """Admit a circuit.  autotune=True first picks the compiled schedule's tiling for this plan by
+        measurement (dcs_amd.autotune; cached per plan fingerprint, optionally in tune_cache)."""
         if name in self.circuits:
             raise ValueError(f'circuit {name!r} already registered')
-        engine = self._factory(graph, unitaries, n, order, parity_operations, open_vertices)
+        args = (graph, unitaries, n, order, parity_operations, open_vertices)
+        if autotune:
+            from .autotune import autotune as tune
+            shared = {k: v for k, v in alloc_kw.items() if k in ('lds_storage', 'pass_max_dim', 'resident_bits')}
+            res = tune(self._factory, args, dtype=dtype, device=self.device, cache_path=tune_cache, alloc_kw=shared)
+            alloc_kw = {**alloc_kw, **res['best'], 'schedule': 'compiled'}
+        engine = self._factory(*args)
         left = self.budget - self.used
         lanes = max(1, int(lanes))
         while lanes >= 1 and footprint_bytes(engine.bp, dtype, lanes, max_batch) > left:

@@ -41,3 +41,30 @@ def test_mixed_serving_equals_alone():
         h = draw_checked(alone, r.uniform)
         assert h['A_index'] == r.index and h['cdf_lower_numpy'] == d['cdf_lower_numpy']
     srv.close()
+
+
+def test_autotuned_schedule_is_correct():
+    """The tuner times three tilings of a width-20 plan on the device and the engine built with the
+    winner stays within 1e-5 of the complex128 oracle."""
+    import numpy as np
+    from dcs_amd.autotune import autotune
+    from dcs_amd.block_plan import BlockPlan
+    from dcs_amd.conditional_block import InplaceConditionalBlockGPU
+    from dcs_amd.stabilizer import synthetic_instance
+    from oracle import oracle
+    print('case autotune w20', flush=True)
+    inst = synthetic_instance(28, 32, 64, seed=0)
+    args = (inst['graph'], inst['unitaries'], 28, inst['natural_order'], [], list(range(20, 28)))
+    x = np.random.default_rng(11).integers(0, 2, 28, dtype=np.uint8)
+    res = autotune(InplaceConditionalBlockGPU, args, x=x, candidates=(
+        {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 1},
+        {'fuse_max_dim': 6, 'tile_log2': 11, 'group_policy': 0, 'lds_buffers': 2},
+        {'fuse_max_dim': 9, 'tile_log2': 13, 'group_policy': 1, 'lds_buffers': 1}))
+    print(res['best'], [round(r['seconds'] * 1e3, 3) for r in res['table']], flush=True)
+    assert all(r['seconds'] and r['seconds'] > 0 for r in res['table'])
+    e = InplaceConditionalBlockGPU(*args)
+    e.alloc(dtype=np.complex64, **res['best'])
+    v = np.asarray(e.block(x)[0])
+    e.close()
+    ref = oracle.block(BlockPlan(*args), x)
+    assert np.max(np.abs(v - ref)) <= 1e-5 * np.max(np.abs(ref))
