@@ -19,7 +19,7 @@ MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED, MODE_COMPILED = 0, 1, 2, 3
 EXPORTED = (
     'dcs_last_error', 'dcs_get_device_count', 'dcs_get_device_info', 'dcs_context_create',
     'dcs_context_destroy', 'dcs_synchronize', 'dcs_plan_create', 'dcs_gather_plan_create', 'dcs_plan_destroy',
-    'dcs_plan_compile', 'dcs_plan_memory',
+    'dcs_plan_compile', 'dcs_plan_memory', 'dcs_plan_use_graphs',
     'dcs_plan_units',
     'dcs_plan_schedule', 'dcs_contract', 'dcs_block_zero', 'dcs_block_accumulate', 'dcs_block_transform',
     'dcs_block_read', 'dcs_block_device_pointer', 'dcs_draw', 'dcs_frontier_read', 'dcs_plan_profile',
@@ -96,6 +96,7 @@ def load():
         'dcs_draw': (C.c_int, [vp, C.c_int, dbl, P(DrawResult)]),
         'dcs_frontier_read': (C.c_int, [vp, C.c_int, vp]),
         'dcs_plan_profile': (C.c_int, [vp, C.c_int, P(dbl), P(C.c_int64)]),
+        'dcs_plan_use_graphs': (C.c_int, [vp, C.c_int]),
         'dcs_stream_copy': (C.c_int, [vp, u64, C.c_int, P(dbl)]),
     }
     for name, (res, args) in sig.items():
@@ -260,6 +261,10 @@ class Plan:
         sec = C.c_double(0.0)
         check(self.lib.dcs_plan_compile(self.handle, C.byref(sec)))
         return sec.value
+
+    def use_graphs(self, enable=True):
+        """Replay contract()'s launch sequence from captured HIP graphs (one per mode and lane count)."""
+        check(self.lib.dcs_plan_use_graphs(self.handle, int(bool(enable))))
 
     def device_bytes(self):
         b = C.c_uint64(0)

@@ -247,7 +247,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
               fuse_max_dim=8, resident_bits=12, tile_log2=12, blocks=1, exact_rounding=False,
-              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=1):
+              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=1, graphs=False):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
@@ -268,6 +268,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    HBM bytes (dynamic programming)
         lds_buffers  compiled schedule: 1 (default) one LDS tile per workgroup, the next tile waiting in
                    registers; 2 double-buffered tiles
+        graphs     replay each slice's launch sequence from a captured HIP graph (one per lane count)
         The legacy fp64_kernel_path argument is accepted and ignored, as in the reference."""
         dt = np.dtype(dtype)
         if dt not in (np.dtype(np.complex64), np.dtype(np.complex128)):
@@ -295,6 +296,9 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                                  'group_policy': group_policy, 'lds_buffers': lds_buffers,
                                  'lds_storage': bool(lds_storage), 'exact_rounding': bool(exact_rounding)}
         self.compile_seconds = self._plan.compile() if self.mode == _native.MODE_COMPILED else 0.0
+        if graphs:
+            self._plan.use_graphs(True)
+        self.schedule_options['graphs'] = bool(graphs)
         self.buf = True
 
     # ---- reference mixin hooks -------------------------------------------------------------

@@ -707,6 +707,9 @@ struct dcs_plan {
     double prof_ms[5] = {0, 0, 0, 0, 0};
     int64_t prof_n[5] = {0, 0, 0, 0, 0};
     std::vector<PendingEvent> pending;
+    // captured launch sequences of dcs_contract, one per (mode, lane count)
+    bool use_graphs = false;
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
 };
 
 namespace {
@@ -1380,6 +1383,8 @@ int run_contract(dcs_plan *p, int mode, int nl, std::vector<u64> &ntiles_of_grou
     return DCS_OK;
 }
 
+int enqueue_contract(dcs_plan *p, int mode, int nl, int nunits_out);
+
 }  // namespace
 
 // ================================================================================================
@@ -1448,6 +1453,8 @@ void dcs_plan_destroy(dcs_plan *p) {
     if (!p) return;
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
+    for (auto &kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+    p->graphs.clear();
     for (auto &pe : p->pending) {
         (void)hipEventDestroy(pe.a);
         (void)hipEventDestroy(pe.b);
@@ -1817,6 +1824,64 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
         }
     for (int i = 0; i < nl; ++i) p->h_lanes[i] = lanes[i];
     hipStream_t st = p->ctx->stream;
+    const int nunits_out = (mode >= DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
+    // Graph replay: the whole sequence below (parameter upload from the pinned staging buffers, maxima
+    // cleared by a fill kernel, every launch, maxima read back into pinned memory) is captured once per
+    // (mode, lane count) and replayed; the staging buffers are read when a replay runs.
+    const bool graphable = p->use_graphs && !p->profiling && p->executor == 0 && mode != DCS_MODE_STEPS_HOST;
+    if (graphable) {
+        const auto key = std::make_pair(mode, nl);
+        auto it = p->graphs.find(key);
+        if (it == p->graphs.end()) {
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_contract(p, mode, nl, nunits_out);
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ec != hipSuccess) return fail(DCS_ERR_RUNTIME, "graph capture: %s", hipGetErrorString(ec));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) return fail(DCS_ERR_RUNTIME, "graph instantiate: %s", hipGetErrorString(ei));
+            it = p->graphs.emplace(key, exec).first;
+        }
+        HIPCHK(hipGraphLaunch(it->second, st));
+        HIPCHK(hipStreamSynchronize(st));
+        memcpy(unit_m2, p->h_dmax, sizeof(double) * (size_t)nl * nunits_out);
+        return DCS_OK;
+    }
+    int rc = enqueue_contract(p, mode, nl, nunits_out);
+    if (rc) {
+        (void)hipStreamSynchronize(st);
+        return rc;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    memcpy(unit_m2, p->h_dmax, sizeof(double) * (size_t)nl * nunits_out);
+    return ev_resolve(p);
+}
+
+int dcs_plan_use_graphs(dcs_plan *p, int enable) {
+    if (!p) return fail(DCS_ERR_INVALID, "null plan");
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->ctx->stream));
+    if (!enable) {
+        for (auto &kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+        p->graphs.clear();
+    }
+    p->use_graphs = enable != 0;
+    return DCS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int enqueue_contract(dcs_plan *p, int mode, int nl, int nunits_out) {
+    const int n = p->nsteps;
+    hipStream_t st = p->ctx->stream;
     HIPCHK(hipMemcpyAsync(p->d_dyn, p->h_dyn, sizeof(Dyn) * (size_t)nl * n, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(p->d_lanes, p->h_lanes, sizeof(int) * nl, hipMemcpyHostToDevice, st));
     int rc;
@@ -1828,16 +1893,14 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
         rc = run_contract<double2>(p, mode, nl, p->group_ntiles);
     else
         rc = run_contract<h16x2>(p, mode, nl, p->group_ntiles);
-    if (rc) {
-        (void)hipStreamSynchronize(st);
-        return rc;
-    }
-    const int nunits = (mode >= DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
-    HIPCHK(hipMemcpyAsync(p->h_dmax, p->d_dmax, sizeof(u64) * (size_t)nl * nunits, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    memcpy(unit_m2, p->h_dmax, sizeof(double) * (size_t)nl * nunits);
-    return ev_resolve(p);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(p->h_dmax, p->d_dmax, sizeof(u64) * (size_t)nl * nunits_out, hipMemcpyDeviceToHost, st));
+    return DCS_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int dcs_block_zero(dcs_plan *p, int block) {
     if (!p || block < 0 || block >= p->blocks) return fail(DCS_ERR_INVALID, "block out of range");

@@ -180,6 +180,13 @@ int dcs_plan_schedule(dcs_plan *plan, int mode, int cap, int32_t *seg_kind, int3
 int dcs_contract(dcs_plan *plan, int mode, int nlanes, const int32_t *lanes, const double *coeffs,
                  const int32_t *ov, const int32_t *y, double *unit_m2);
 
+/* 1: dcs_contract captures its launch sequence (parameter upload, maxima clear, every kernel of the
+ * schedule, maxima read-back) into a HIP graph once per (mode, nlanes) and replays it on later calls;
+ * 0 (default): launches every call, and frees the captured graphs.  DCS_MODE_STEPS_HOST and profiling
+ * always launch directly.  No counterpart in the reference (CuPy launches every step from Python,
+ * kernel_opt/inplace_gpu.py:246-259); it serves callers that contract many prefixes of one plan. */
+int dcs_plan_use_graphs(dcs_plan *plan, int enable);
+
 int dcs_block_zero(dcs_plan *plan, int block);
 /* total[smap[i] ^ xmask] += env_lane(i) * coef, env at the terminal slots of the lane. */
 int dcs_block_accumulate(dcs_plan *plan, int block, int lane, uint64_t xmask, double coef_re,
