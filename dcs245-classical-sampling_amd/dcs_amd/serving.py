@@ -12,7 +12,11 @@ process keeps several circuits resident on the GPU and serves a stream of reques
     requests together (slices of all of them share lane launches, block_many), without waiting for a
     batch to fill;
   * every block is bit-identical to serving its request alone, and so is its draw (block_many);
-  * per-request latency (arrival to draw) gives p50 / p99 per circuit.
+  * per-request latency (arrival to draw) gives p50 / p99 per circuit;
+  * serve(concurrent=True) runs one worker thread per resident circuit, each batching its own queue on
+    its engine's HIP stream (default priority), so requests for different circuits are on the GPU at
+    the same time (the library releases the interpreter lock during device calls) and a short request
+    no longer waits behind another circuit's batch; arrivals are released in real time.
 
 `poisson_workload` generates seeded arrivals (exponential gaps) over a weighted circuit mix, with
 prefixes and uniforms drawn from PCG64 in the order of the reference protocol (sampling_protocol.py:21-33).
@@ -22,6 +26,7 @@ from __future__ import annotations
 import collections
 import json
 import math
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -115,13 +120,16 @@ class Server:
             self.unregister(name)
 
     # ---- the loop ------------------------------------------------------------------------------
-    def serve(self, requests, realtime=True, keep_blocks=False):
+    def serve(self, requests, realtime=True, keep_blocks=False, concurrent=False):
         """Serve requests (any order; each has an arrival time in seconds from now).  With realtime=False
         idle gaps are skipped instead of slept through (latencies are then measured on a clock that
-        jumps to the next arrival when nothing is waiting).  Returns the requests, filled in."""
+        jumps to the next arrival when nothing is waiting).  concurrent=True serves every circuit from
+        its own worker thread (arrivals in real time).  Returns the requests, filled in."""
         for r in requests:
             if r.circuit not in self.circuits:
                 raise KeyError(f'request {r.rid}: unknown circuit {r.circuit!r}')
+        if concurrent:
+            return self._serve_concurrent(requests, keep_blocks)
         todo = sorted(requests, key=lambda r: (r.arrival, r.rid))
         queues = {name: collections.deque() for name in self.circuits}
         t0 = time.perf_counter()
@@ -144,16 +152,69 @@ class Server:
             q = min(waiting, key=lambda q: (q[0].arrival, q[0].rid))
             c = self.circuits[q[0].circuit]
             batch = [q.popleft() for _ in range(min(c.max_batch, len(q)))]
-            start = clock()
-            vecs, _ = c.engine.block_many([r.x_bits for r in batch])
-            for r, v in zip(batch, vecs):
-                r.draw = v.draw(r.uniform)
-                if keep_blocks:
-                    r.block = np.asarray(v)
-                r.index = r.draw['A_index']
-                r.start, r.done, r.batch = start, clock(), nbatch
-            c.served += len(batch)
+            self._run_batch(c, batch, clock, nbatch, keep_blocks)
             nbatch += 1
+        return requests
+
+
+    def _run_batch(self, c, batch, clock, nbatch, keep_blocks):
+        start = clock()
+        vecs, _ = c.engine.block_many([r.x_bits for r in batch])
+        for r, v in zip(batch, vecs):
+            r.draw = v.draw(r.uniform)
+            if keep_blocks:
+                r.block = np.asarray(v)
+            r.index = r.draw['A_index']
+            r.start, r.done, r.batch = start, clock(), nbatch
+        c.served += len(batch)
+
+    def _serve_concurrent(self, requests, keep_blocks):
+        todo = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        cv = threading.Condition()
+        queues = {name: collections.deque() for name in self.circuits}
+        state = {'released': 0, 'batches': 0, 'error': None}
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0
+
+        def worker(name):
+            c = self.circuits[name]
+            while True:
+                with cv:
+                    cv.wait_for(lambda: queues[name] or state['released'] == len(todo) or state['error'])
+                    if state['error'] or (not queues[name] and state['released'] == len(todo)):
+                        return
+                    q = queues[name]
+                    batch = [q.popleft() for _ in range(min(c.max_batch, len(q)))]
+                    nb = state['batches']
+                    state['batches'] += 1
+                try:
+                    self._run_batch(c, batch, clock, nb, keep_blocks)
+                except BaseException as exc:     # stop every worker, re-raise in the caller
+                    with cv:
+                        state['error'] = exc
+                        cv.notify_all()
+                    return
+
+        threads = [threading.Thread(target=worker, args=(name,), daemon=True) for name in self.circuits]
+        for t in threads:
+            t.start()
+        for r in todo:
+            wait = r.arrival - clock()
+            if wait > 0:
+                time.sleep(wait)
+            with cv:
+                if state['error']:
+                    break
+                queues[r.circuit].append(r)
+                state['released'] += 1
+                cv.notify_all()
+        with cv:
+            state['released'] = len(todo)
+            cv.notify_all()
+        for t in threads:
+            t.join()
+        if state['error'] is not None:
+            raise state['error']
         return requests
 
 
@@ -212,6 +273,7 @@ def main(argv=None):
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--no-ibm', action='store_true')
     ap.add_argument('--realtime', action='store_true')
+    ap.add_argument('--concurrent', action='store_true', help='one worker thread per circuit (real-time arrivals)')
     a = ap.parse_args(argv)
     from .workloads import load_ibm64
     srv = Server()
@@ -233,10 +295,11 @@ def main(argv=None):
     warm = [Request(-1 - j, name, np.zeros(mix[name][1], np.uint8), 0.5, 0.0) for j, name in enumerate(mix)]
     srv.serve(warm, realtime=False)
     reqs = poisson_workload(mix, a.rate, a.requests, a.seed)
-    srv.serve(reqs, realtime=a.realtime)
+    srv.serve(reqs, realtime=a.realtime or a.concurrent, concurrent=a.concurrent)
     stats = latency_stats(reqs)
     line = {'metric': 'serving: continuous batching over resident circuits, p50/p99 ms per circuit',
-            'requests': a.requests, 'rate_per_s': a.rate, 'seed': a.seed, 'realtime': a.realtime,
+            'requests': a.requests, 'rate_per_s': a.rate, 'seed': a.seed, 'realtime': a.realtime or a.concurrent,
+            'concurrent': a.concurrent,
             'setup_s': setup, 'hbm_used_bytes': srv.used, 'hbm_budget_bytes': srv.budget,
             'circuits': {c.name: {'lanes': c.lanes, 'max_batch': c.max_batch, 'device_bytes': c.nbytes,
                                   'rmax': c.engine.inplace_rmax, 'served': c.served} for c in srv.circuits.values()},

@@ -10,8 +10,11 @@ from dcs_amd.stabilizer import synthetic_instance
 pytestmark = pytest.mark.gpu
 
 
-def test_mixed_serving_equals_alone():
-    print('case serving mix', flush=True)
+@pytest.mark.parametrize('concurrent', [False, True])
+def test_mixed_serving_equals_alone(concurrent):
+    """Sequential loop, and one worker thread per circuit (the circuits' batches in flight together on
+    their own streams)."""
+    print(f'case serving mix concurrent={concurrent}', flush=True)
     a = synthetic_instance(12, 10, 14, seed=1)
     b = synthetic_instance(16, 12, 20, seed=0)
     c = synthetic_instance(28, 32, 64, seed=0)
@@ -26,7 +29,7 @@ def test_mixed_serving_equals_alone():
         srv.register(name, *args, lanes=lanes, max_batch=3, schedule='compiled', lds_storage=True)
         mix[name] = (1, args[2], args[5])
     reqs = poisson_workload(mix, rate=200.0, count=24, seed=5)
-    srv.serve(reqs, realtime=False, keep_blocks=True)
+    srv.serve(reqs, realtime=False, keep_blocks=True, concurrent=concurrent)
     st = latency_stats(reqs)
     print({k: (v['count'], round(v['p50_ms'], 2), round(v['p99_ms'], 2)) for k, v in st.items()}, flush=True)
     assert st['all']['count'] == 24

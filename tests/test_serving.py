@@ -89,6 +89,55 @@ def test_continuous_batching_order_and_bounds():
     assert st['all']['count'] == 6 and st['small']['count'] == 4 and st['all']['p99_ms'] >= st['all']['p50_ms']
 
 
+class SlowEngine(FakeEngine):
+    """Stand-in whose batches take time proportional to the width (released interpreter lock)."""
+    active, peak = 0, 0
+
+    def block_many(self, xs):
+        import time
+        SlowEngine.active += 1
+        SlowEngine.peak = max(SlowEngine.peak, SlowEngine.active)
+        time.sleep(0.002 * (self.inplace_rmax - 9))
+        SlowEngine.active -= 1
+        return super().block_many(xs)
+
+
+def test_concurrent_serving_overlaps_circuits_and_keeps_results():
+    SlowEngine.log, SlowEngine.active, SlowEngine.peak = [], 0, 0
+    srv = Server(hbm_budget=1 << 50, engine_factory=SlowEngine)
+    srv.register('small', 10, None, 8, None, [], [0, 1], lanes=4, max_batch=2)
+    srv.register('big', 40, None, 8, None, [], [0, 1], lanes=1, max_batch=2)
+    rng = np.random.default_rng(3)
+    reqs = [Request(i, 'big' if i % 4 == 0 else 'small', rng.integers(0, 2, 8, dtype=np.uint8), float(rng.random()),
+                    0.005 * i) for i in range(24)]
+    want = {r.rid: r.x_bits.sum() * 1000 + int(r.uniform * 1000) for r in reqs}
+    srv.serve(reqs, concurrent=True)
+    assert all(r.index == want[r.rid] and r.done >= r.arrival >= 0 for r in reqs)
+    assert SlowEngine.peak == 2            # a small batch ran while a big one was in flight
+    for name in ('small', 'big'):          # FIFO per circuit, batches within max_batch
+        mine = sorted((r for r in reqs if r.circuit == name), key=lambda r: r.arrival)
+        assert [r.batch for r in mine] == sorted(r.batch for r in mine)
+        assert max(collections_count(r.batch for r in mine).values()) <= 2
+    st = latency_stats(reqs)
+    assert st['small']['p99_ms'] < st['big']['p50_ms'] + 100   # small requests do not wait behind big ones
+
+
+def collections_count(it):
+    import collections
+    return collections.Counter(it)
+
+
+def test_concurrent_serving_propagates_engine_errors():
+    class Broken(FakeEngine):
+        def block_many(self, xs):
+            raise ArithmeticError('numerically zero block')
+    srv = Server(hbm_budget=1 << 40, engine_factory=Broken)
+    srv.register('a', 10, None, 8, None, [], [0, 1], lanes=1, max_batch=2)
+    x = np.zeros(8, np.uint8)
+    with pytest.raises(ArithmeticError):
+        srv.serve([Request(i, 'a', x, .5, 0.0) for i in range(3)], concurrent=True)
+
+
 def test_poisson_workload_is_seeded_and_weighted():
     mix = {'a': (3, 10, [8, 9]), 'b': (1, 12, [0])}
     r1 = poisson_workload(mix, rate=50.0, count=400, seed=7)
