@@ -6,7 +6,8 @@ replayed from the frozen PCG64 protocol (its first samples).  One step = one sam
 every slice contraction, the coherent slice sum, the Walsh-Hadamard / phase / Clifford tail and the
 device CDF draw.  Inputs (graph, plan, frontier buffer) are resident before the timed region.
 
-Fast path (value): complex64 storage, plan inplace_k2_fp32 (4 slices, rmax 34, 128 GiB frontier),
+Fast path (value): complex64 storage, plan inplace_k2_fp32 (4 slices, rmax 34, 128 GiB frontier per
+lane; two prefixes at a time by default, their slices sharing each launch),
 compiled schedule (LDS-resident runs + one generated kernel per coset group, complex64 LDS tiles;
 coset groups of dimension <= 8 on 2^12-entry tiles with the fewest-HBM-bytes group boundaries).
 Plain path (reported beside it): complex128
@@ -60,40 +61,45 @@ def fused_bytes_per_sample(engine, elem):
     return {k: v * nslices for k, v in out.items()}
 
 
-def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False, **alloc_kw):
+def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False, lanes=1, **alloc_kw):
+    """Contract nsamples prefixes `lanes` at a time (block_many: the slices of `lanes` prefixes share
+    each launch), after `warmup` untimed ones.  With lanes > 1 a request's latency is its batch's."""
     from dcs_amd.conditional_block import InplaceConditionalBlockGPU
     from dcs_amd.workloads import load_ibm64
     w = load_ibm64()
     e = InplaceConditionalBlockGPU(w['graph'], w['unitaries'], 64, w['order'], w['plans'][plan_name]['operations'],
                                    w['open_vertices'])
     t0 = time.perf_counter()
-    e.alloc(dtype=dtype, schedule=schedule, **alloc_kw)
+    e.alloc(dtype=dtype, schedule=schedule, lanes=lanes, blocks=lanes, **alloc_kw)
     setup = time.perf_counter() - t0
-    work = prefixes(warmup + nsamples)
-    for x, u in work[:warmup]:
-        vec, _ = e.block(x)
-        vec.draw(u)
+    work = prefixes(warmup + nsamples + lanes)
+
+    def run(batch):
+        vecs, _ = e.block_many([x for x, _ in batch])
+        return [v.draw(u)['A_index'] for v, (_, u) in zip(vecs, batch)]
+    for i in range(0, warmup, lanes):
+        run(work[i:min(warmup, i + lanes)])
     e.ctx.synchronize()
     lat = []
     draws = []
     t0 = time.perf_counter()
-    for x, u in work[warmup:]:
+    for i in range(warmup, warmup + nsamples, lanes):
+        batch = work[i:min(warmup + nsamples, i + lanes)]
         s = time.perf_counter()
-        vec, meta = e.block(x)
-        d = vec.draw(u)
-        lat.append(time.perf_counter() - s)
-        draws.append(d['A_index'])
+        draws += run(batch)
+        lat += [time.perf_counter() - s] * len(batch)
     e.ctx.synchronize()
     elapsed = time.perf_counter() - t0
     prof = None
     if profile:
-        # one more sample outside the timed region with HIP events around every launch
+        # one more batch outside the timed region with HIP events around every launch, per sample
         e._plan.profile(True)
-        x, u = work[-1]
-        vec, _ = e.block(x)
-        vec.draw(u)
-        prof = e._plan.profile(False)
-    info = {'engine': e, 'options': dict(e.schedule_options), 'elapsed': elapsed, 'lat': lat, 'setup': setup, 'draws': draws, 'profile': prof}
+        run(work[warmup + nsamples:warmup + nsamples + lanes])
+        ms, n = e._plan.profile(False)
+        prof = (ms / lanes, n / lanes)
+    info = {'engine': e, 'options': dict(e.schedule_options), 'elapsed': elapsed, 'lat': lat, 'setup': setup,
+            'draws': draws, 'profile': prof, 'lanes': lanes,
+            'samples_run': warmup + nsamples + (lanes if profile else 0)}
     return info
 
 
@@ -145,6 +151,8 @@ def main():
     ap.add_argument('--plain-samples', type=int, default=1)
     ap.add_argument('--schedule', default='compiled', choices=['compiled', 'fused', 'steps', 'steps_host'])
     ap.add_argument('--lds-fp64', action='store_true', help='compiled schedule: FP64 LDS tiles')
+    ap.add_argument('--lanes', type=int, default=2,
+                    help='prefixes contracted together (frontiers held at once); falls back to 1 if they do not fit')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plain', action='store_true')
     a = ap.parse_args()
@@ -166,7 +174,11 @@ def main():
 
     log('fast path: complex64, k2 plan, schedule', a.schedule)
     kw = {'lds_storage': not a.lds_fp64} if a.schedule == 'compiled' else {}
-    fast = run_path('inplace_k2_fp32', np.complex64, a.schedule, a.steps, a.warmup, profile=True, **kw)
+    try:
+        fast = run_path('inplace_k2_fp32', np.complex64, a.schedule, a.steps, a.warmup, profile=True, lanes=a.lanes, **kw)
+    except MemoryError as exc:
+        log(f'{a.lanes} lanes do not fit ({exc}); one lane')
+        fast = run_path('inplace_k2_fp32', np.complex64, a.schedule, a.steps, a.warmup, profile=True, lanes=1, **kw)
     eng = fast['engine']
     ms, launches = fast['profile']
     sps = a.steps / fast['elapsed']
@@ -207,7 +219,8 @@ def main():
     if tf.exists():
         rec = json.loads(tf.read_text())
         # counter-derived bytes belong to one schedule: use them only for the same launches and bytes
-        if rec.get('launches_per_sample') == int(launches[2]) and rec.get('algorithmic_bytes_per_sample') == coset_bytes:
+        if (rec.get('launches_per_sample') == float(launches[2]) and rec.get('lanes_per_launch', 1) == fast['lanes']
+                and rec.get('algorithmic_bytes_per_sample') == coset_bytes):
             traffic = rec.get('hbm_bytes_per_launch')
     lat = np.array(fast['lat']) * 1e3
     line = {
@@ -222,8 +235,9 @@ def main():
                    'schedule': a.schedule, 'lds_tiles': ('fp64' if a.lds_fp64 else 'storage') if a.schedule == 'compiled' else None,
                    'fusion': {k: fast['options'][k] for k in ('fuse_max_dim', 'tile_log2', 'pass_max_dim', 'group_policy',
                                                                'lds_buffers')},
+                   'batch': fast['lanes'],
                    'block': 1 << 20},
-        'setup_s': fast['setup'],
+        'setup_s': fast['setup'], 'fast_samples_run': fast['samples_run'],
         'effective_hbm_GBps': unfused_bytes * sps / 1e9,
         'effective_roofline_frac': unfused_bytes * sps / HBM_PEAK,
         'algorithmic_bytes_per_sample': {'unfused': unfused_bytes, 'fused': sum(fb.values())},
@@ -232,7 +246,7 @@ def main():
         'speedup_vs_plain_c128': (sps / plain['samples_per_s']) if plain else None,
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9 if achieved else None, 'peak': HBM_PEAK / 1e9,
                      'unit': 'GB/s', 'frac': (achieved / HBM_PEAK) if achieved else None, 'traffic': traffic,
-                     'launches_per_sample': int(launches[2]),
+                     'launches_per_sample': float(launches[2]), 'lanes_per_launch': fast['lanes'],
                      'kernel': kernel, 'bytes_per_sample': coset_bytes, 'ms_per_sample': coset_ms},
         'cpu_baseline': cpu,
         'device': {k: dev[k] for k in ('device_name', 'arch', 'compute_units', 'total_bytes')},
