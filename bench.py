@@ -74,11 +74,19 @@ def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False, lanes=
     setup = time.perf_counter() - t0
     work = prefixes(warmup + nsamples + lanes)
 
-    def run(batch):
+    kept = {}
+
+    def run(batch, keep=False):
         vecs, _ = e.block_many([x for x, _ in batch])
-        return [v.draw(u)['A_index'] for v, (_, u) in zip(vecs, batch)]
+        out = []
+        for v, (_, u) in zip(vecs, batch):
+            d = v.draw(u)
+            if keep and not kept:     # the first prefix's block and draw, for the parity check
+                kept.update(block=np.asarray(v), draw=d)
+            out.append(d['A_index'])
+        return out
     for i in range(0, warmup, lanes):
-        run(work[i:min(warmup, i + lanes)])
+        run(work[i:min(warmup, i + lanes)], keep=True)
     e.ctx.synchronize()
     lat = []
     draws = []
@@ -86,7 +94,7 @@ def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False, lanes=
     for i in range(warmup, warmup + nsamples, lanes):
         batch = work[i:min(warmup + nsamples, i + lanes)]
         s = time.perf_counter()
-        draws += run(batch)
+        draws += run(batch, keep=warmup == 0 and i == 0)
         lat += [time.perf_counter() - s] * len(batch)
     e.ctx.synchronize()
     elapsed = time.perf_counter() - t0
@@ -98,9 +106,22 @@ def run_path(plan_name, dtype, schedule, nsamples, warmup, profile=False, lanes=
         ms, n = e._plan.profile(False)
         prof = (ms / lanes, n / lanes)
     info = {'engine': e, 'options': dict(e.schedule_options), 'elapsed': elapsed, 'lat': lat, 'setup': setup,
-            'draws': draws, 'profile': prof, 'lanes': lanes,
+            'draws': draws, 'profile': prof, 'lanes': lanes, 'first': kept,
             'samples_run': warmup + nsamples + (lanes if profile else 0)}
     return info
+
+
+def first_prefix_parity(fast, plain, uniform, tie=1e-7):
+    """The fast path's complex64 block of the first protocol prefix against the plain complex128 block
+    of the same prefix (different slice plans: one global phase aligned), and the two draws."""
+    a, r = fast['block'], plain['block']
+    ratio = np.vdot(r, a) / np.vdot(r, r)
+    err = float(np.max(np.abs(a - (ratio / abs(ratio)) * r)) / np.max(np.abs(r)))
+    lo, hi = plain['draw']['cdf_lower_numpy'], plain['draw']['cdf_upper_numpy']
+    margin = min(uniform - lo, hi - uniform)
+    same = fast['draw']['A_index'] == plain['draw']['A_index']
+    return {'prefixes': 1, 'max_rel_amp_error_vs_c128': err, 'draw_identical': same,
+            'cdf_edge_margin': margin, 'tie': bool(margin < tie)}
 
 
 def cpu_baseline(max_width=32):
@@ -194,6 +215,7 @@ def main():
     del eng, fast['engine']
 
     plain = None
+    parity = None
     if not a.no_plain and a.plain_samples > 0:
         log('plain path: complex128, k3 plan, per-step launches with host maxima')
         p = run_path('inplace_k3_fp64', np.complex128, 'steps_host', a.plain_samples, 0)
@@ -205,6 +227,9 @@ def main():
                  'roofline_frac': pbytes * a.plain_samples / p['elapsed'] / HBM_PEAK}
         log('plain', plain)
         p['engine'].close()
+        if fast['first'] and p['first']:
+            parity = first_prefix_parity(fast['first'], p['first'], prefixes(1)[0][1])
+            log('parity', parity)
 
     cpu = None
     if not a.no_cpu:
@@ -244,6 +269,7 @@ def main():
         'stream_copy_GBps': stream_GBps,
         'plain_c128': plain,
         'speedup_vs_plain_c128': (sps / plain['samples_per_s']) if plain else None,
+        'parity_first_prefix': parity,
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9 if achieved else None, 'peak': HBM_PEAK / 1e9,
                      'unit': 'GB/s', 'frac': (achieved / HBM_PEAK) if achieved else None, 'traffic': traffic,
                      'launches_per_sample': float(launches[2]), 'lanes_per_launch': fast['lanes'],
