@@ -824,6 +824,10 @@ std::vector<int> group_ends_from(const dcs_plan *p, int s, int D) {
 // while the span stays <= D).  Policy 1: the partition with the fewest HBM bytes, by dynamic
 // programming over group ends: a group reads its live entries once and writes them once, so its cost
 // is (2^|live before| + 2^|live after|) entries plus a fixed launch allowance of 2^20 entries.
+// Policy 2: the same dynamic programme with the register passes priced too: a group runs each of its
+// steps over every entry of its tile span, live or not, 2^|slots live anywhere in the group| per step,
+// weighted 0.31 against 1.73 per entry moved (the measured cost of an element-step and of an HBM entry
+// of the generated kernels on MI355X, from compute-only and memory-only variants).
 void choose_groups(const dcs_plan *p, int D, std::vector<int> &group_end) {
     const int n = p->nsteps;
     group_end.assign(n, -1);
@@ -856,11 +860,17 @@ void choose_groups(const dcs_plan *p, int D, std::vector<int> &group_end) {
             std::vector<int> from(b - s + 1, -1);
             best[0] = 0;
             auto live = [&](u64 m) { return std::ldexp(1.0, popc(m)); };
+            const double w_mem = p->group_policy == 2 ? 1.73 : 1.0, w_cmp = p->group_policy == 2 ? 0.31 : 0.0;
             for (int i = s; i < b; ++i) {
                 if (best[i - s] >= inf) continue;
+                u64 span = p->steps[i].in_mask;
+                int last = i;
                 for (int e : group_ends_from(p, i, D)) {
-                    const double c = best[i - s] + live(p->steps[i].in_mask) + live(p->steps[e - 1].out_mask) +
-                                     std::ldexp(1.0, 20);
+                    for (; last < e; ++last) span |= p->steps[last].out_mask;
+                    const double c = best[i - s] +
+                                     w_mem * (live(p->steps[i].in_mask) + live(p->steps[e - 1].out_mask) +
+                                              std::ldexp(1.0, 20)) +
+                                     w_cmp * (double)(e - i) * live(span);
                     if (c < best[e - s]) {
                         best[e - s] = c;
                         from[e - s] = i;
@@ -1481,7 +1491,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
         return fail(DCS_ERR_INVALID, "terminal mask holds more than k slots (k=%d)", d->k);
     if (d->tile_log2 < 11 || d->tile_log2 > 13)
         return fail(DCS_ERR_INVALID, "tile_log2 must be 11 or 12 (13: DCS_MODE_COMPILED only)");
-    if (d->group_policy < 0 || d->group_policy > 1) return fail(DCS_ERR_INVALID, "group_policy must be 0 or 1");
+    if (d->group_policy < 0 || d->group_policy > 2) return fail(DCS_ERR_INVALID, "group_policy must be 0, 1 or 2");
     if (d->lds_buffers < 0 || d->lds_buffers > 2) return fail(DCS_ERR_INVALID, "lds_buffers must be 0, 1 or 2");
     if (d->resident_bits != 0 && (d->resident_bits < 6 || d->resident_bits > 12))
         return fail(DCS_ERR_INVALID, "resident_bits must be 0 or 6..12");
