@@ -173,7 +173,11 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
         auto u = [&](int E) { return par32(TE[E] & q.bvec); };
         for (int m = 0; m < OPT; ++m) {
             emitf(o, "      const int sb%d = (__popc(t%d & %uu) + __popc(l%d & %uu) + sg) & 1;\n", m, m, q.bvec, m, q.beta_lane);
-            emitf(o, "      const cd F%d = sb%d ? cd{-q.f1r, -q.f1i} : cd{q.f1r, q.f1i};\n", m, m);
+            // +-f1 by flipping the sign bits (bit-identical to negation): no divergent branch around the
+            // parameter loads
+            emitf(o, "      const long long sm%d = (long long)((unsigned long long)sb%d << 63);\n", m, m);
+            emitf(o, "      const cd F%d{__longlong_as_double(__double_as_longlong(q.f1r) ^ sm%d), "
+                     "__longlong_as_double(__double_as_longlong(q.f1i) ^ sm%d)};\n", m, m, m);
             if (q.kind == 2 || q.kind == 7)
                 emitf(o, "      const cd W%d_0 = cadd(f0, F%d), W%d_1 = csub(f0, F%d);\n", m, m, m, m);
             if (q.kind == 4) emitf(o, "      const cd C%d_0 = q.y ? F%d : f0, C%d_1 = q.y ? cneg(F%d) : f0;\n", m, m, m, m);
