@@ -80,9 +80,8 @@ struct Store<double2> {
 
 // NaN-propagating maximum of non-negative values (a NaN anywhere must reach the host check).
 __device__ __forceinline__ double max_nan(double a, double b) {
-    if (a != a) return a;
-    if (b != b) return b;
-    return a > b ? a : b;
+    const double m = a > b ? a : b;
+    return (a != a || b != b) ? a + b : m;   // a + b is NaN whenever either operand is; no branches
 }
 
 // Scale of the next unit from the previous maximum |out|^2:  2^-e with e = frexp(sqrt(m2)).exp,

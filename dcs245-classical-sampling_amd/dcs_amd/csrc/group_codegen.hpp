@@ -96,10 +96,10 @@ __device__ __forceinline__ cd cfma2(cd p, cd a, cd q, cd b) {
     i = fma(p.i, a.r, i);
     return cd{r, i};
 }
+// NaN-propagating maximum without branches: a + b is NaN whenever either operand is
 __device__ __forceinline__ float max_nan_f(float a, float b) {
-    if (a != a) return a;
-    if (b != b) return b;
-    return a > b ? a : b;
+    const float m = a > b ? a : b;
+    return (a != a || b != b) ? a + b : m;
 }
 )";
     return o;
