@@ -2,7 +2,8 @@
 
 The compiled schedule has four knobs that trade HBM bytes against register passes and occupancy:
 fuse_max_dim (largest coset dimension of a group), tile_log2 (entries per LDS tile), group_policy
-(greedy, fewest-bytes or bytes-and-passes boundaries) and lds_buffers (one or two LDS tiles per workgroup).  Their best
+(greedy, fewest-bytes or bytes-and-passes boundaries) and lds_buffers (one LDS tile per workgroup, two,
+or two filled by asynchronous global-to-LDS loads).  Their best
 setting depends on the plan's width profile, so for a given plan the tuner allocates an engine per
 candidate, times its contraction of one prefix (after a warm-up contraction, with the schedule's own
 device timers summed over the group, resident and per-step kernels), and keeps the fastest.  Results
@@ -19,13 +20,13 @@ from pathlib import Path
 import numpy as np
 
 DEFAULT_CANDIDATES = (
+    {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 3},
     {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 1},
-    {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 2, 'lds_buffers': 1},
-    {'fuse_max_dim': 9, 'tile_log2': 13, 'group_policy': 2, 'lds_buffers': 1},
+    {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 2, 'lds_buffers': 3},
+    {'fuse_max_dim': 9, 'tile_log2': 13, 'group_policy': 2, 'lds_buffers': 3},
     {'fuse_max_dim': 9, 'tile_log2': 13, 'group_policy': 1, 'lds_buffers': 1},
-    {'fuse_max_dim': 7, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 1},
-    {'fuse_max_dim': 6, 'tile_log2': 11, 'group_policy': 1, 'lds_buffers': 1},
-    {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 2},
+    {'fuse_max_dim': 7, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 3},
+    {'fuse_max_dim': 6, 'tile_log2': 11, 'group_policy': 1, 'lds_buffers': 3},
     {'fuse_max_dim': 6, 'tile_log2': 11, 'group_policy': 0, 'lds_buffers': 2},
 )
 

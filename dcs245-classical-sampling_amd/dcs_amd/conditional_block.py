@@ -247,7 +247,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
               fuse_max_dim=8, resident_bits=12, tile_log2=12, blocks=1, exact_rounding=False,
-              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=1, graphs=False):
+              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=3, graphs=False):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
@@ -266,8 +266,11 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    defaults (8, 12, 3) are the fastest measured on the 64-qubit plans
         group_policy  0: greedy maximal coset groups; 1 (default): the group boundaries with the fewest
                    HBM bytes (dynamic programming)
-        lds_buffers  compiled schedule: 1 (default) one LDS tile per workgroup, the next tile waiting in
-                   registers; 2 double-buffered tiles
+        lds_buffers  compiled schedule: 1 one LDS tile per workgroup, the next tile waiting in
+                   registers; 2 double-buffered tiles; 3 (default) two tiles, the next one loaded
+                   straight from HBM into LDS (global_load_lds) during the current tile's passes, where
+                   a group allows it (complex64 tiles in LDS, at least two lanes), else as 1; the same
+                   arithmetic as 1, so the same blocks bit for bit
         graphs     replay each slice's launch sequence from a captured HIP graph (one per lane count)
         The legacy fp64_kernel_path argument is accepted and ignored, as in the reference."""
         dt = np.dtype(dtype)

@@ -32,7 +32,8 @@ struct CodegenOptions {
     int tile_log2;     // log2 of the largest tile (entries)
     bool exact;        // round to storage precision after every step
     bool lds_storage;  // LDS tiles in storage precision (rounded once per register pass) instead of FP64
-    int lds_buffers;   // LDS tiles per workgroup: 2 (double-buffered) or 1
+    int lds_buffers;   // LDS tiles per workgroup: 2 (double-buffered) or 1; 3: two tiles filled by
+                       // asynchronous global-to-LDS loads where a group allows it (else as 1)
 };
 
 namespace codegen {
@@ -71,6 +72,13 @@ inline std::string prelude(const CodegenOptions &opt, const char *kernels_hpp) {
     const char *lt = opt.storage == 2 ? (opt.lds_storage ? "float2" : "double2")
                                       : ((opt.lds_storage || opt.storage == 1) ? st : "double2");
     emitf(o, "typedef %s St;\ntypedef %s Lt;\n", st, lt);
+    o += "typedef __attribute__((address_space(1))) const void *dcs_gptr;\n"
+         "typedef __attribute__((address_space(3))) void *dcs_lptr;\n";
+    // Workgroup barrier that waits for this wave's LDS instructions only.  __syncthreads() carries a
+    // workgroup release fence, which also waits for global-to-LDS loads still in flight; between the
+    // register passes of the asynchronous-load pipeline those target the other tile, which nobody
+    // reads before the explicit wait after the passes.
+    o += "__device__ __forceinline__ void dcs_lds_barrier() { asm volatile(\"s_waitcnt lgkmcnt(0)\\n\\ts_barrier\" ::: \"memory\"); }\n";
     o += R"(
 __device__ __forceinline__ cd cneg(cd a) { return cd{-a.r, -a.i}; }
 template <int U> __device__ __forceinline__ cd sgn(cd a) { if constexpr (U) return cneg(a); else return a; }
@@ -142,7 +150,10 @@ inline u64 lane_dep_const(const GroupDev &g, uint32_t l) {
 }
 
 // One register pass.  `T` names the current LDS buffer pointer in the generated code.
-inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView &v, const PassDev &ps, int threads) {
+// m0: coset coordinates whose address carries bit 0; the entry (t, l) then sits at LDS index
+// (t W + l) ^ parity(t & m0), a linear relabelling (zero outside the asynchronous-load mode).
+inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView &v, const PassDev &ps, int threads,
+                      uint32_t m0 = 0) {
     const GroupDev &g = *v.g;
     const int J = ps.nmask, NE = 1 << J, WB = g.wbits, D = g.d;
     const uint32_t norb = (1u << (D - J)) << WB;
@@ -161,8 +172,9 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
         emitf(o, "    const uint32_t l%d = o%d & %uu;\n", m, m, (1u << WB) - 1);
         emitf(o, "    uint32_t t%d = o%d >> %d;\n", m, m, WB);
         for (int i = 0; i < J; ++i) emitf(o, "    t%d = insert0(t%d, %d);\n", m, m, ps.piv[i]);
-        emitf(o, "    const uint32_t b%d = (t%d << %d) | l%d;\n", m, m, WB, m);
-        for (int E = 0; E < NE; ++E) emitf(o, "    cd x%d_%d = LG(T[b%d ^ %uu]);\n", m, E, m, TE[E] << WB);
+        if (m0) emitf(o, "    const uint32_t b%d = ((t%d << %d) | l%d) ^ (__popc(t%d & %uu) & 1u);\n", m, m, WB, m, m, m0);
+        else emitf(o, "    const uint32_t b%d = (t%d << %d) | l%d;\n", m, m, WB, m);
+        for (int E = 0; E < NE; ++E) emitf(o, "    cd x%d_%d = LG(T[b%d ^ %uu]);\n", m, E, m, (TE[E] << WB) ^ (uint32_t)par32(TE[E] & m0));
     }
     for (int s = 0; s < ps.nsteps; ++s) {
         const PStepDev &q = v.psteps[ps.first_pstep + s];
@@ -294,7 +306,8 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
         emitf(o, "    }\n");
     }
     for (int m = 0; m < OPT; ++m)
-        for (int E = 0; E < NE; ++E) emitf(o, "    T[b%d ^ %uu] = LP(x%d_%d);\n", m, TE[E] << WB, m, E);
+        for (int E = 0; E < NE; ++E)
+            emitf(o, "    T[b%d ^ %uu] = LP(x%d_%d);\n", m, (TE[E] << WB) ^ (uint32_t)par32(TE[E] & m0), m, E);
     if (norb < (uint32_t)threads) emitf(o, "    }\n");
     emitf(o, "    }\n");
 }
@@ -331,7 +344,21 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
              "dcs_group_%d(St *__restrict__ base, u64 lane_stride,\n", threads, gi);
     o += "    const int *__restrict__ lane_ids, const Dyn *__restrict__ dyn, int nsteps, u64 *__restrict__ dmax, int nunits,\n"
          "    int unit) {\n";
-    const bool single = opt.lds_buffers == 1;
+    // Asynchronous tile loads (lds_buffers 3): every lane moves 16 bytes, two complex64 entries that
+    // are neighbours in memory (the tile row holds the lowest address bits), straight into the other
+    // LDS tile; no registers hold the next tile and there is no put phase.
+    // The pair x, x+1 (x even, lane bit 0 clear) sits at addresses a, a ^ 1: in memory order when a is
+    // even, swapped when the coset part of a carries bit 0 (the tile base never does, bit 0 being a lane
+    // bit).  The LDS index permutation with m0 = the coordinates carrying bit 0 stores both
+    // orientations where the 16-byte load puts them.
+    const bool dma = opt.lds_buffers == 3 && c64 && lds_is_storage && g.lane_identity && WB >= 1 && !guard &&
+                     PER >= 2;
+    uint32_t m0 = 0;
+    if (dma)
+        for (int i = 0; i < D; ++i)
+            if (addr_of(v, 1u << i) & 1ull) m0 |= 1u << i;
+    auto lidx = [&](uint32_t e) { return e ^ (uint32_t)par32((e >> WB) & m0); };
+    const bool single = opt.lds_buffers == 1 || (opt.lds_buffers == 3 && !dma);
     emitf(o, "  __shared__ Lt buf[%u];\n", single ? N : 2 * N);
     emitf(o, "  const u64 NT = %lluull;\n", (unsigned long long)g.ntiles);
     o += "  u64 ti = blockIdx.x;\n  if (ti >= NT) return;\n"
@@ -430,8 +457,10 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         const unsigned long long C = (unsigned long long)Ck[k];
         if (c64 && lds_is_storage) {
             // complex64 values from LDS: the power-of-two scale is exact in float when representable
-            emitf(o, "    if (!tdout%s) { const float2 a = P[%s]; float2 r;\n", gexpr.c_str(),
-                  guard ? "tid" : (std::string("vt + ") + std::to_string(off)).c_str());
+            const std::string at = guard ? std::string("tid")
+                                 : m0 ? std::string("lvt ^ ") + std::to_string(lidx(off)) + "u"
+                                      : std::string("vt + ") + std::to_string(off);
+            emitf(o, "    if (!tdout%s) { const float2 a = P[%s]; float2 r;\n", gexpr.c_str(), at.c_str());
             o += "      if (sfast) r = make_float2(a.x * sf, a.y * sf);\n"
                  "      else r = make_float2((float)((double)a.x * scale), (float)((double)a.y * scale));\n";
             emitf(o, "      E[tbx ^ pth ^ 0x%llxull] = r;\n", C);
@@ -441,6 +470,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
                   guard ? "tid" : (std::string("vt + ") + std::to_string(off)).c_str());
         }
     };
+    if (m0) emitf(o, "  const uint32_t lvt = vt ^ (__popc((vt >> %d) & %uu) & 1u);\n", WB, m0);
     o += "  auto store = [&](Lt *P, u64 tbx) {\n";
     for (int k = 0; k < PER; ++k) store_one(k);
     o += "  };\n";
@@ -456,7 +486,72 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         }
         o += "  };\n";
     }
+    if (dma) {
+        // lane element x = 2 (j threads + tid): address = tile base ^ pa ^ D_j (linear deposit)
+        std::string pa = "0ull";
+        for (int p = 1; p <= EB; ++p) {
+            const u64 c = (p < WB) ? lane_dep_const(g, 1u << p) : addr_of(v, 1u << (p - WB));
+            char b[160];
+            snprintf(b, sizeof(b), " ^ ((0ull - (u64)((vt >> %d) & 1u)) & 0x%llxull)", p - 1, (unsigned long long)c);
+            pa += b;
+        }
+        emitf(o, "  const u64 pa = %s;\n", pa.c_str());
+        emitf(o, "  const bool tdin2 = ((((uint32_t)vt << 1) >> %d) & %uu) != 0;\n", WB, g.dead_in_t);
+        // dead entries are zero-filled first: an LDS store after a global-to-LDS load into the same
+        // tile would make the compiler wait for the load
+        o += "  auto dma_fill = [&](Lt *P, u64 tbx) {\n";
+        for (int j = 0; j < PER / 2; ++j) {
+            const uint32_t x = 2u * (uint32_t)j * span;
+            const uint32_t t = x >> WB;
+            if (t & g.dead_in_t)
+                emitf(o, "    P[%uu + 2u * vt] = Store<St>::zero(); P[%uu + 2u * vt + 1u] = Store<St>::zero();\n", x, x);
+            else
+                emitf(o, "    if (tdin2) { P[%uu + 2u * vt] = Store<St>::zero(); P[%uu + 2u * vt + 1u] = Store<St>::zero(); }\n",
+                      x, x);
+        }
+        for (int j = 0; j < PER / 2; ++j) {
+            const uint32_t x = 2u * (uint32_t)j * span;
+            const uint32_t t = x >> WB, l = x & (W - 1);
+            const unsigned long long Dj = (unsigned long long)(lane_dep_const(g, l) ^ addr_of(v, t));
+            if (!(t & g.dead_in_t))
+                emitf(o, "    if (!tdin2) __builtin_amdgcn_global_load_lds((dcs_gptr)(E + ((tbx ^ pa ^ 0x%llxull) & ~1ull)), "
+                         "(dcs_lptr)(P + %uu + 2u * (vt & ~63u)), 16, 0, 0);\n", Dj, x);
+        }
+        o += "  };\n";
+        // vmcnt(0) with the other counters at their maximum (gfx9 encoding): every load into LDS landed
+        o += "  auto dma_wait = [&]() { __builtin_amdgcn_s_waitcnt(0x0F70); };\n";
+    }
     // pipeline
+    if (dma) {
+        // the tile body takes the two tiles as restrict pointers: the loads into U and the passes'
+        // accesses to T get distinct alias scopes, so no wait for the loads is placed before the passes
+        o += "  u64 tb = tile_base(ti);\n";
+        o += "  auto passes = [&](Lt *__restrict__ T, Lt *__restrict__ U, bool more, u64 tn) {\n"
+             "    if (more) dma_fill(U, tile_base(tn));\n";
+        for (int pi = 0; pi < g.npasses; ++pi) {
+            emit_pass(o, opt, v, v.passes[pi], threads, m0);
+            if (pi + 1 < g.npasses) o += "    dcs_lds_barrier();\n";
+        }
+        o += "  };\n";
+        o += "  dma_fill(buf, tb);\n  dma_wait();\n  __syncthreads();\n"
+             "  int c = 0;\n"
+             "  for (;;) {\n";
+        emitf(o, "    Lt *T = buf + c * %u;\n    Lt *U = buf + (c ^ 1) * %u;\n", N, N);
+        o += "    const u64 tn = ti + gridDim.x;\n"
+             "    const bool more = tn < NT;\n"
+             "    passes(T, U, more, tn);\n";
+        o += "    dma_wait();\n"
+             "    __syncthreads();\n"
+             "    store(T, tb);\n"
+             "    if (!more) break;\n"
+             "    __syncthreads();\n"
+             "    c ^= 1;\n    ti = tn;\n    tb = tile_base(ti);\n"
+             "  }\n";
+        if (c64 && lds_is_storage) o += "  commit_max((double)lmf, &dmax[(u64)li * nunits + unit]);\n";
+        else o += "  commit_max(lm, &dmax[(u64)li * nunits + unit]);\n";
+        o += "}\n";
+        return o;
+    }
     emitf(o, "  u64 tb = tile_base(ti);\n  issue(tb);\n  put(buf);\n  __syncthreads();\n");
     if (single) {
         o += "  for (;;) {\n"

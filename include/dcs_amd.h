@@ -108,7 +108,10 @@ typedef struct {
                                    HBM bytes (dynamic programming over the step sequence); 2: the same with
                                    the register passes over each group's span priced as well */
     int32_t lds_buffers;        /* DCS_MODE_COMPILED: LDS tiles per workgroup, 2 (double-buffered, the
-                                   default when 0) or 1 (the next tile waits in registers) */
+                                   default when 0) or 1 (the next tile waits in registers); 3: two
+                                   tiles, the next one loaded from HBM straight into LDS
+                                   (global_load_lds) during the current one's passes, in groups whose
+                                   complex64 tiles allow it (else as 1); same results as 1 */
 } dcs_plan_desc;
 
 typedef struct {

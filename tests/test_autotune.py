@@ -51,7 +51,7 @@ def test_autotune_picks_fastest_and_caches(args, tmp_path):
     FakeEngine.allocs.clear()
     cache = tmp_path / 'tune.json'
     res = at.autotune(FakeEngine, args, arch='gfx950', cache_path=cache)
-    assert res['best'] == {'fuse_max_dim': 7, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 1}
+    assert res['best'] == {'fuse_max_dim': 7, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 3}
     assert not res['cached'] and len(res['table']) == len(at.DEFAULT_CANDIDATES)
     refused = [r for r in res['table'] if r['seconds'] is None]
     assert len(refused) == sum(c['tile_log2'] == 13 for c in at.DEFAULT_CANDIDATES)

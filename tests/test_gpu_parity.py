@@ -122,13 +122,16 @@ def test_w20_compiled_matches_oracle(w20, tl, pj, lds):
 
 
 @pytest.mark.parametrize('tl,dim,policy,buffers', [(11, 6, 1, 1), (12, 8, 1, 1), (12, 9, 1, 2), (13, 9, 1, 1),
-                                                  (11, 12, 1, 2), (12, 12, 1, 1), (12, 8, 2, 1), (13, 9, 2, 1)])
+                                                  (11, 12, 1, 2), (12, 12, 1, 1), (12, 8, 2, 1), (13, 9, 2, 1),
+                                                  (12, 8, 1, 3), (11, 6, 0, 3), (13, 9, 2, 3)])
 def test_w20_wide_groups_match_oracle(w20, tl, dim, policy, buffers):
     """Coset groups of up to 2^12 entries by dimension, on 2^11..2^13 tiles, with the fewest-bytes group
     boundaries (which put slots born or dying inside a group on coordinates above other pivots) and one
-    or two LDS tiles per workgroup: complex128 within 1e-13 of the oracle, per-step-rounded complex64
-    within 1e-13 of the oracle's complex64 (and equal to the generic coset kernel where that kernel
-    supports the tile), production complex64 no further from FP64 than the per-step schedule (x2)."""
+    or two LDS tiles per workgroup, or two filled by asynchronous global-to-LDS loads (buffers 3):
+    complex128 within 1e-13 of the oracle, per-step-rounded complex64 within 1e-13 of the oracle's
+    complex64 (and equal to the generic coset kernel where that kernel supports the tile), production
+    complex64 no further from FP64 than the per-step schedule (x2); asynchronous loads change no
+    arithmetic, so that block equals the one-tile pipeline's bit for bit."""
     print(f'case w20 wide tl={tl} dim={dim} policy={policy} buffers={buffers}', flush=True)
     kw = dict(schedule='compiled', tile_log2=tl, fuse_max_dim=dim, group_policy=policy, lds_buffers=buffers)
     e = engine(w20['args'], dtype=np.complex128, **kw)
@@ -149,6 +152,10 @@ def test_w20_wide_groups_match_oracle(w20, tl, dim, policy, buffers):
     e.close()
     err, err_steps = rel(v, w20['ref128']), rel(w20['ref64'], w20['ref128'])
     assert err < 1e-5 and err <= 2 * err_steps, (err, err_steps)
+    if buffers == 3:
+        e = engine(w20['args'], dtype=np.complex64, **{**kw, 'lds_buffers': 1})
+        assert np.array_equal(np.asarray(e.block(w20['x'])[0]), v)
+        e.close()
 
 
 def test_c1_matches_statevector_and_64_draws_seed0(c1):
