@@ -104,11 +104,8 @@ __device__ __forceinline__ cd cfma2(cd p, cd a, cd q, cd b) {
     i = fma(p.i, a.r, i);
     return cd{r, i};
 }
-// NaN-propagating maximum without branches: a + b is NaN whenever either operand is
-__device__ __forceinline__ float max_nan_f(float a, float b) {
-    const float m = a > b ? a : b;
-    return (a != a || b != b) ? a + b : m;
-}
+// NaN-propagating maximum (llvm.maximum: one v_maximum3_f32 on gfx950)
+__device__ __forceinline__ float max_nan_f(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 )";
     return o;
 }
@@ -451,7 +448,9 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         }
     }
     o += "  };\n";
-    auto store_one = [&](int k) {
+    // fm: 1 the float product (scale exact in float), 2 the FP64 product; the caller branches on the
+    // wave-uniform sfast once around a whole store instead of once per entry
+    auto store_one = [&](int k, int fm) {
         if (dout_k[k]) return;
         const uint32_t off = (uint32_t)k * span;
         const unsigned long long C = (unsigned long long)Ck[k];
@@ -461,8 +460,8 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
                                  : m0 ? std::string("lvt ^ ") + std::to_string(lidx(off)) + "u"
                                       : std::string("vt + ") + std::to_string(off);
             emitf(o, "    if (!tdout%s) { const float2 a = P[%s]; float2 r;\n", gexpr.c_str(), at.c_str());
-            o += "      if (sfast) r = make_float2(a.x * sf, a.y * sf);\n"
-                 "      else r = make_float2((float)((double)a.x * scale), (float)((double)a.y * scale));\n";
+            if (fm == 1) o += "      r = make_float2(a.x * sf, a.y * sf);\n";
+            else o += "      r = make_float2((float)((double)a.x * scale), (float)((double)a.y * scale));\n";
             emitf(o, "      E[tbx ^ pth ^ 0x%llxull] = r;\n", C);
             o += "      lmf = max_nan_f(lmf, r.x * r.x + r.y * r.y); }\n";
         } else {
@@ -471,19 +470,32 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         }
     };
     if (m0) emitf(o, "  const uint32_t lvt = vt ^ (__popc((vt >> %d) & %uu) & 1u);\n", WB, m0);
+    const bool split = c64 && lds_is_storage;
     o += "  auto store = [&](Lt *P, u64 tbx) {\n";
-    for (int k = 0; k < PER; ++k) store_one(k);
+    if (split) {
+        o += "   if (sfast) {\n";
+        for (int k = 0; k < PER; ++k) store_one(k, 1);
+        o += "   } else {\n";
+        for (int k = 0; k < PER; ++k) store_one(k, 2);
+        o += "   }\n";
+    } else {
+        for (int k = 0; k < PER; ++k) store_one(k, 0);
+    }
     o += "  };\n";
     if (single) {
         // each thread reads its finished entry, stores it, then writes the loaded entry at the same
         // LDS address: per-thread program order keeps the read before the write
         o += "  auto store_put = [&](Lt *P, u64 tbx) {\n";
-        for (int k = 0; k < PER; ++k) {
-            store_one(k);
-            const uint32_t off = (uint32_t)k * span;
-            if (guard) emitf(o, "    if (tid < %uu) P[tid] = LP(Store<St>::load(R[%d]));\n", N, k);
-            else emitf(o, "    P[vt + %u] = LP(Store<St>::load(R[%d]));\n", off, k);
+        for (int fm : split ? std::vector<int>{1, 2} : std::vector<int>{0}) {
+            if (split) o += fm == 1 ? "   if (sfast) {\n" : "   } else {\n";
+            for (int k = 0; k < PER; ++k) {
+                store_one(k, fm);
+                const uint32_t off = (uint32_t)k * span;
+                if (guard) emitf(o, "    if (tid < %uu) P[tid] = LP(Store<St>::load(R[%d]));\n", N, k);
+                else emitf(o, "    P[vt + %u] = LP(Store<St>::load(R[%d]));\n", off, k);
+            }
         }
+        if (split) o += "   }\n";
         o += "  };\n";
     }
     if (dma) {

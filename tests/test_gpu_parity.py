@@ -272,9 +272,10 @@ def test_canonical_parity_slices_and_prefixes():
     e.close()
 
 
-def test_nonfinite_prefix_raises(c1):
-    print('case nonfinite', flush=True)
-    e = engine(c1['args'], dtype=np.complex64, schedule='fused')
+@pytest.mark.parametrize('sched', ['fused', 'compiled'])
+def test_nonfinite_prefix_raises(c1, sched):
+    print(f'case nonfinite {sched}', flush=True)
+    e = engine(c1['args'], dtype=np.complex64, schedule=sched)
     w = {q: np.array([np.nan, 1.0]) for q in range(16, 36)}
     with pytest.raises(ArithmeticError):
         e.block(c1['x'], w=w)
