@@ -162,6 +162,19 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
             if ((E >> l) & 1) t ^= ps.mw[l];
         TE[E] = t;
     }
+    // mask over the orbit number o whose parity is parity(t & bvec) ^ parity(l & beta_lane)
+    auto orbit_sign_mask = [&](uint32_t bvec, uint32_t beta_lane) {
+        uint32_t mx = 0;
+        for (int j = 0; j < D - J; ++j) {
+            uint32_t pos = 1u << j;
+            for (int i = 0; i < J; ++i) {
+                const uint32_t lo = pos & ((1u << ps.piv[i]) - 1);
+                pos = ((pos ^ lo) << 1) | lo;  // insert0(pos, piv[i])
+            }
+            if (pos & bvec) mx |= 1u << j;
+        }
+        return (mx << WB) | (beta_lane & ((1u << WB) - 1));
+    };
     emitf(o, "    { // pass: span %d, %d steps\n", J, ps.nsteps);
     if (norb < (uint32_t)threads) emitf(o, "    if (tid < %uu) {\n", norb);
     for (int m = 0; m < OPT; ++m) {
@@ -181,7 +194,10 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
         emitf(o, "      const int sg = (__popcll(tb & 0x%llxull) + q.ov) & 1;\n", (unsigned long long)q.beta);
         auto u = [&](int E) { return par32(TE[E] & q.bvec); };
         for (int m = 0; m < OPT; ++m) {
-            emitf(o, "      const int sb%d = (__popc(t%d & %uu) + __popc(l%d & %uu) + sg) & 1;\n", m, m, q.bvec, m, q.beta_lane);
+            // parity(t & bvec) + parity(l & beta_lane) is parity(o & M): t and l are bit moves of the
+            // orbit number o (l its low WB bits, t the rest with zeros inserted at the pass pivots).
+            // Only bit 0 of sb is used.
+            emitf(o, "      const int sb%d = __popc(o%d & %uu) + sg;\n", m, m, orbit_sign_mask(q.bvec, q.beta_lane));
             // +-f1 by flipping the sign bits (bit-identical to negation): no divergent branch around the
             // parameter loads
             emitf(o, "      const long long sm%d = (long long)((unsigned long long)sb%d << 63);\n", m, m);
