@@ -14,6 +14,8 @@
 //   run the register passes on the current tile (LDS), then move the loaded tile into the other LDS
 //   buffer.  Stores and loads are in flight during the passes; the only wait on global memory comes
 //   after them.
+// Waves raise their priority (s_setprio) while they issue a tile's loads and stores and drop it for the
+// register passes, so that memory work of one wave is not held back behind the passes of others.
 // One LDS tile (half the LDS, twice the workgroups per CU): issue the next tile's loads, run the
 //   passes, then each thread stores its finished entries and puts the loaded ones in their place
 //   (the same LDS addresses, so no barrier is needed between the two); the stores drain during the
@@ -555,7 +557,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         // accesses to T get distinct alias scopes, so no wait for the loads is placed before the passes
         o += "  u64 tb = tile_base(ti);\n";
         o += "  auto passes = [&](Lt *__restrict__ T, Lt *__restrict__ U, bool more, u64 tn) {\n"
-             "    if (more) dma_fill(U, tile_base(tn));\n";
+             "    if (more) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tile_base(tn)); __builtin_amdgcn_s_setprio(0); }\n";
         for (int pi = 0; pi < g.npasses; ++pi) {
             emit_pass(o, opt, v, v.passes[pi], threads, m0);
             if (pi + 1 < g.npasses) o += "    dcs_lds_barrier();\n";
@@ -570,7 +572,9 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
              "    passes(T, U, more, tn);\n";
         o += "    dma_wait();\n"
              "    __syncthreads();\n"
+             "    __builtin_amdgcn_s_setprio(2);\n"
              "    store(T, tb);\n"
+             "    __builtin_amdgcn_s_setprio(0);\n"
              "    if (!more) break;\n"
              "    __syncthreads();\n"
              "    c ^= 1;\n    ti = tn;\n    tb = tile_base(ti);\n"
@@ -586,13 +590,15 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
              "    Lt *T = buf;\n"
              "    const u64 tn = ti + gridDim.x;\n"
              "    const bool more = tn < NT;\n"
-             "    if (more) issue(tile_base(tn));\n";
+             "    if (more) { __builtin_amdgcn_s_setprio(2); issue(tile_base(tn)); __builtin_amdgcn_s_setprio(0); }\n";
         for (int pi = 0; pi < g.npasses; ++pi) {
             emit_pass(o, opt, v, v.passes[pi], threads);
             o += "    __syncthreads();\n";
         }
-        o += "    if (!more) { store(T, tb); break; }\n"
+        o += "    __builtin_amdgcn_s_setprio(2);\n"
+             "    if (!more) { store(T, tb); break; }\n"
              "    store_put(T, tb);\n"
+             "    __builtin_amdgcn_s_setprio(0);\n"
              "    __syncthreads();\n"
              "    ti = tn;\n    tb = tile_base(ti);\n"
              "  }\n";
