@@ -528,8 +528,11 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         emitf(o, "  const u64 pa = %s;\n", pa.c_str());
         emitf(o, "  const bool tdin2 = ((((uint32_t)vt << 1) >> %d) & %uu) != 0;\n", WB, g.dead_in_t);
         // dead entries are zero-filled first: an LDS store after a global-to-LDS load into the same
-        // tile would make the compiler wait for the load
-        o += "  auto dma_fill = [&](Lt *P, u64 tbx) {\n";
+        // tile would make the compiler wait for the load.  part 1: the zero fills and the first half of
+        // the loads (issued before the first register pass), part 2: the second half (issued after it,
+        // which spreads a workgroup's requests over the tile), part 0: everything
+        o += "  auto dma_fill = [&](Lt *P, u64 tbx, int part) {\n";
+        o += "   if (part != 2) {\n";
         for (int j = 0; j < PER / 2; ++j) {
             const uint32_t x = 2u * (uint32_t)j * span;
             const uint32_t t = x >> WB;
@@ -539,7 +542,10 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
                 emitf(o, "    if (tdin2) { P[%uu + 2u * vt] = Store<St>::zero(); P[%uu + 2u * vt + 1u] = Store<St>::zero(); }\n",
                       x, x);
         }
+        o += "   }\n";
         for (int j = 0; j < PER / 2; ++j) {
+            if (j == 0) o += "   if (part != 2) {\n";
+            if (j == PER / 4) o += "   }\n   if (part != 1) {\n";
             const uint32_t x = 2u * (uint32_t)j * span;
             const uint32_t t = x >> WB, l = x & (W - 1);
             const unsigned long long Dj = (unsigned long long)(lane_dep_const(g, l) ^ addr_of(v, t));
@@ -547,6 +553,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
                 emitf(o, "    if (!tdin2) __builtin_amdgcn_global_load_lds((dcs_gptr)(E + ((tbx ^ pa ^ 0x%llxull) & ~1ull)), "
                          "(dcs_lptr)(P + %uu + 2u * (vt & ~63u)), 16, 0, 0);\n", Dj, x);
         }
+        o += "   }\n";
         o += "  };\n";
         // vmcnt(0) with the other counters at their maximum (gfx9 encoding): every load into LDS landed
         o += "  auto dma_wait = [&]() { __builtin_amdgcn_s_waitcnt(0x0F70); };\n";
@@ -557,13 +564,15 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         // accesses to T get distinct alias scopes, so no wait for the loads is placed before the passes
         o += "  u64 tb = tile_base(ti);\n";
         o += "  auto passes = [&](Lt *__restrict__ T, Lt *__restrict__ U, bool more, u64 tn) {\n"
-             "    if (more) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tile_base(tn)); __builtin_amdgcn_s_setprio(0); }\n";
+             "    const u64 tbn = more ? tile_base(tn) : 0ull;\n"
+             "    if (more) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tbn, 1); __builtin_amdgcn_s_setprio(0); }\n";
         for (int pi = 0; pi < g.npasses; ++pi) {
             emit_pass(o, opt, v, v.passes[pi], threads, m0);
+            if (pi == 0) o += "    if (more) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tbn, 2); __builtin_amdgcn_s_setprio(0); }\n";
             if (pi + 1 < g.npasses) o += "    dcs_lds_barrier();\n";
         }
         o += "  };\n";
-        o += "  dma_fill(buf, tb);\n  dma_wait();\n  __syncthreads();\n"
+        o += "  dma_fill(buf, tb, 0);\n  dma_wait();\n  __syncthreads();\n"
              "  int c = 0;\n"
              "  for (;;) {\n";
         emitf(o, "    Lt *T = buf + c * %u;\n    Lt *U = buf + (c ^ 1) * %u;\n", N, N);
