@@ -9,11 +9,15 @@
 // per-step formulas are those of dcs_kernels.hpp (orb_*), with the same operand order, so a step's
 // result is bit-identical to the per-step kernel's before rounding to storage.
 //
-// Pipeline of one workgroup (persistent over the tiles of its group), two LDS tiles:
-//   store the previous tile's results (LDS -> HBM), issue the next tile's loads (HBM -> registers),
-//   run the register passes on the current tile (LDS), then move the loaded tile into the other LDS
-//   buffer.  Stores and loads are in flight during the passes; the only wait on global memory comes
-//   after them.
+// Pipelines of one workgroup (persistent over the tiles of its group):
+// Two LDS tiles filled by global-to-LDS loads (lds_buffers 3, complex64 tiles): run the first register
+//   pass on the current tile, store the previous tile's results from the other LDS tile, synchronise,
+//   issue the next tile's loads into that tile (HBM -> LDS, no registers), run the remaining passes,
+//   then wait for the loads.
+// Two LDS tiles filled from registers (lds_buffers 2): store the previous tile's results (LDS -> HBM),
+//   issue the next tile's loads (HBM -> registers), run the register passes on the current tile (LDS),
+//   then move the loaded tile into the other LDS buffer.  Stores and loads are in flight during the
+//   passes; the only wait on global memory comes after them.
 // Waves raise their priority (s_setprio) while they issue a tile's loads and stores and drop it for the
 // register passes, so that memory work of one wave is not held back behind the passes of others.
 // One LDS tile (half the LDS, twice the workgroups per CU): issue the next tile's loads, run the
@@ -528,11 +532,8 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         emitf(o, "  const u64 pa = %s;\n", pa.c_str());
         emitf(o, "  const bool tdin2 = ((((uint32_t)vt << 1) >> %d) & %uu) != 0;\n", WB, g.dead_in_t);
         // dead entries are zero-filled first: an LDS store after a global-to-LDS load into the same
-        // tile would make the compiler wait for the load.  part 1: the zero fills and the first half of
-        // the loads (issued before the first register pass), part 2: the second half (issued after it,
-        // which spreads a workgroup's requests over the tile), part 0: everything
-        o += "  auto dma_fill = [&](Lt *P, u64 tbx, int part) {\n";
-        o += "   if (part != 2) {\n";
+        // tile would make the compiler wait for the load
+        o += "  auto dma_fill = [&](Lt *P, u64 tbx) {\n";
         for (int j = 0; j < PER / 2; ++j) {
             const uint32_t x = 2u * (uint32_t)j * span;
             const uint32_t t = x >> WB;
@@ -542,10 +543,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
                 emitf(o, "    if (tdin2) { P[%uu + 2u * vt] = Store<St>::zero(); P[%uu + 2u * vt + 1u] = Store<St>::zero(); }\n",
                       x, x);
         }
-        o += "   }\n";
         for (int j = 0; j < PER / 2; ++j) {
-            if (j == 0) o += "   if (part != 2) {\n";
-            if (j == PER / 4) o += "   }\n   if (part != 1) {\n";
             const uint32_t x = 2u * (uint32_t)j * span;
             const uint32_t t = x >> WB, l = x & (W - 1);
             const unsigned long long Dj = (unsigned long long)(lane_dep_const(g, l) ^ addr_of(v, t));
@@ -553,7 +551,6 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
                 emitf(o, "    if (!tdin2) __builtin_amdgcn_global_load_lds((dcs_gptr)(E + ((tbx ^ pa ^ 0x%llxull) & ~1ull)), "
                          "(dcs_lptr)(P + %uu + 2u * (vt & ~63u)), 16, 0, 0);\n", Dj, x);
         }
-        o += "   }\n";
         o += "  };\n";
         // vmcnt(0) with the other counters at their maximum (gfx9 encoding): every load into LDS landed
         o += "  auto dma_wait = [&]() { __builtin_amdgcn_s_waitcnt(0x0F70); };\n";
@@ -561,31 +558,36 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
     // pipeline
     if (dma) {
         // the tile body takes the two tiles as restrict pointers: the loads into U and the passes'
-        // accesses to T get distinct alias scopes, so no wait for the loads is placed before the passes
+        // accesses to T get distinct alias scopes, so no wait for the loads is placed before the passes.
+        // U holds the previous tile's results until they are stored: each wave stores them after its
+        // part of the first register pass (mixing the stores with other waves' FP64 work), and the next
+        // tile's loads into U follow the barrier that ends that pass.
         o += "  u64 tb = tile_base(ti);\n";
-        o += "  auto passes = [&](Lt *__restrict__ T, Lt *__restrict__ U, bool more, u64 tn) {\n"
+        o += "  auto passes = [&](Lt *__restrict__ T, Lt *__restrict__ U, bool more, u64 tn, bool prev, u64 tprev) {\n"
              "    const u64 tbn = more ? tile_base(tn) : 0ull;\n"
-             "    if (more) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tbn, 1); __builtin_amdgcn_s_setprio(0); }\n";
+             "    if (more && !prev) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tbn); __builtin_amdgcn_s_setprio(0); }\n";
         for (int pi = 0; pi < g.npasses; ++pi) {
             emit_pass(o, opt, v, v.passes[pi], threads, m0);
-            if (pi == 0) o += "    if (more) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tbn, 2); __builtin_amdgcn_s_setprio(0); }\n";
-            if (pi + 1 < g.npasses) o += "    dcs_lds_barrier();\n";
+            if (pi == 0) {
+                o += "    if (prev) { __builtin_amdgcn_s_setprio(2); store(U, tprev); __builtin_amdgcn_s_setprio(0); }\n"
+                     "    dcs_lds_barrier();\n"
+                     "    if (more && prev) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tbn); __builtin_amdgcn_s_setprio(0); }\n";
+            } else if (pi + 1 < g.npasses) {
+                o += "    dcs_lds_barrier();\n";
+            }
         }
         o += "  };\n";
-        o += "  dma_fill(buf, tb, 0);\n  dma_wait();\n  __syncthreads();\n"
-             "  int c = 0;\n"
+        o += "  dma_fill(buf, tb);\n  dma_wait();\n  __syncthreads();\n"
+             "  int c = 0;\n  bool prev = false;\n  u64 tprev = 0;\n"
              "  for (;;) {\n";
         emitf(o, "    Lt *T = buf + c * %u;\n    Lt *U = buf + (c ^ 1) * %u;\n", N, N);
         o += "    const u64 tn = ti + gridDim.x;\n"
              "    const bool more = tn < NT;\n"
-             "    passes(T, U, more, tn);\n";
+             "    passes(T, U, more, tn, prev, tprev);\n";
         o += "    dma_wait();\n"
              "    __syncthreads();\n"
-             "    __builtin_amdgcn_s_setprio(2);\n"
-             "    store(T, tb);\n"
-             "    __builtin_amdgcn_s_setprio(0);\n"
-             "    if (!more) break;\n"
-             "    __syncthreads();\n"
+             "    if (!more) { __builtin_amdgcn_s_setprio(2); store(T, tb); break; }\n"
+             "    prev = true;\n    tprev = tb;\n"
              "    c ^= 1;\n    ti = tn;\n    tb = tile_base(ti);\n"
              "  }\n";
         if (c64 && lds_is_storage) o += "  commit_max((double)lmf, &dmax[(u64)li * nunits + unit]);\n";
