@@ -2,7 +2,7 @@
 
 The compiled schedule has four knobs that trade HBM bytes against register passes and occupancy:
 fuse_max_dim (largest coset dimension of a group), tile_log2 (entries per LDS tile), group_policy
-(greedy, fewest-bytes or bytes-and-passes boundaries) and lds_buffers (one LDS tile per workgroup, two,
+(greedy, fewest-bytes, bytes-and-passes summed or overlapped boundaries) and lds_buffers (one LDS tile per workgroup, two,
 or two filled by asynchronous global-to-LDS loads).  Their best
 setting depends on the plan's width profile, so for a given plan the tuner allocates an engine per
 candidate, times its contraction of one prefix (after a warm-up contraction, with the schedule's own
@@ -23,6 +23,7 @@ DEFAULT_CANDIDATES = (
     {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 3},
     {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 1},
     {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 2, 'lds_buffers': 3},
+    {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 3, 'lds_buffers': 3},
     {'fuse_max_dim': 9, 'tile_log2': 13, 'group_policy': 2, 'lds_buffers': 3},
     {'fuse_max_dim': 9, 'tile_log2': 13, 'group_policy': 1, 'lds_buffers': 1},
     {'fuse_max_dim': 7, 'tile_log2': 12, 'group_policy': 1, 'lds_buffers': 3},

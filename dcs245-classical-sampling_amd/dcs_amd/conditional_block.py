@@ -265,7 +265,9 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    of its LDS tile (11, 12; 13 compiled only), span of one register pass (3, 4); the
                    defaults (8, 12, 3) are the fastest measured on the 64-qubit plans
         group_policy  0: greedy maximal coset groups; 1 (default): the group boundaries with the fewest
-                   HBM bytes (dynamic programming)
+                   HBM bytes (dynamic programming); 2: bytes and register-pass work summed; 3: the
+                   same two costs overlapped, max plus a quarter of the smaller (both calibrated on
+                   complex64 tiles)
         lds_buffers  compiled schedule: 1 one LDS tile per workgroup, the next tile waiting in
                    registers; 2 double-buffered tiles; 3 (default) two tiles, the next one loaded
                    straight from HBM into LDS (global_load_lds) during the current tile's passes, where

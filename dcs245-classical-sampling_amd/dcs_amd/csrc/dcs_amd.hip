@@ -860,17 +860,22 @@ void choose_groups(const dcs_plan *p, int D, std::vector<int> &group_end) {
             std::vector<int> from(b - s + 1, -1);
             best[0] = 0;
             auto live = [&](u64 m) { return std::ldexp(1.0, popc(m)); };
-            const double w_mem = p->group_policy == 2 ? 1.73 : 1.0, w_cmp = p->group_policy == 2 ? 0.31 : 0.0;
+            // policy 1: HBM entries moved (+ a launch allowance of 2^20 entries); 2: entries at 1.73
+            // and element-steps over the group's span at 0.31 (measured costs, summed); 3: the same two
+            // times overlapped, max(memory, compute) + a quarter of the smaller one (a group's loads and
+            // stores run during its passes, three quarters of the shorter phase hidden as measured)
+            const double w_mem = p->group_policy >= 2 ? 1.73 : 1.0, w_cmp = p->group_policy >= 2 ? 0.31 : 0.0;
+            const bool overlap = p->group_policy == 3;
             for (int i = s; i < b; ++i) {
                 if (best[i - s] >= inf) continue;
                 u64 span = p->steps[i].in_mask;
                 int last = i;
                 for (int e : group_ends_from(p, i, D)) {
                     for (; last < e; ++last) span |= p->steps[last].out_mask;
-                    const double c = best[i - s] +
-                                     w_mem * (live(p->steps[i].in_mask) + live(p->steps[e - 1].out_mask) +
-                                              std::ldexp(1.0, 20)) +
-                                     w_cmp * (double)(e - i) * live(span);
+                    const double tm = w_mem * (live(p->steps[i].in_mask) + live(p->steps[e - 1].out_mask));
+                    const double tc = w_cmp * (double)(e - i) * live(span);
+                    const double c = best[i - s] + w_mem * std::ldexp(1.0, 20) +
+                                     (overlap ? std::max(tm, tc) + 0.25 * std::min(tm, tc) : tm + tc);
                     if (c < best[e - s]) {
                         best[e - s] = c;
                         from[e - s] = i;
@@ -1491,7 +1496,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
         return fail(DCS_ERR_INVALID, "terminal mask holds more than k slots (k=%d)", d->k);
     if (d->tile_log2 < 11 || d->tile_log2 > 13)
         return fail(DCS_ERR_INVALID, "tile_log2 must be 11 or 12 (13: DCS_MODE_COMPILED only)");
-    if (d->group_policy < 0 || d->group_policy > 2) return fail(DCS_ERR_INVALID, "group_policy must be 0, 1 or 2");
+    if (d->group_policy < 0 || d->group_policy > 3) return fail(DCS_ERR_INVALID, "group_policy must be 0..3");
     if (d->lds_buffers < 0 || d->lds_buffers > 3) return fail(DCS_ERR_INVALID, "lds_buffers must be 0..3");
     if (d->resident_bits != 0 && (d->resident_bits < 6 || d->resident_bits > 12))
         return fail(DCS_ERR_INVALID, "resident_bits must be 0 or 6..12");

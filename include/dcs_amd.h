@@ -106,7 +106,8 @@ typedef struct {
                                    (rounded once per register pass), 0 in FP64 */
     int32_t group_policy;       /* 0: greedy maximal coset groups; 1: the group boundaries with the fewest
                                    HBM bytes (dynamic programming over the step sequence); 2: the same with
-                                   the register passes over each group's span priced as well */
+                                   the register passes over each group's span priced as well; 3: those two
+                                   costs overlapped, max(memory, passes) + a quarter of the smaller */
     int32_t lds_buffers;        /* DCS_MODE_COMPILED: LDS tiles per workgroup, 2 (double-buffered, the
                                    default when 0) or 1 (the next tile waits in registers); 3: two
                                    tiles, the next one loaded from HBM straight into LDS

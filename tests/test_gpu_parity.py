@@ -123,7 +123,7 @@ def test_w20_compiled_matches_oracle(w20, tl, pj, lds):
 
 @pytest.mark.parametrize('tl,dim,policy,buffers', [(11, 6, 1, 1), (12, 8, 1, 1), (12, 9, 1, 2), (13, 9, 1, 1),
                                                   (11, 12, 1, 2), (12, 12, 1, 1), (12, 8, 2, 1), (13, 9, 2, 1),
-                                                  (12, 8, 1, 3), (11, 6, 0, 3), (13, 9, 2, 3)])
+                                                  (12, 8, 1, 3), (11, 6, 0, 3), (13, 9, 2, 3), (12, 8, 3, 3)])
 def test_w20_wide_groups_match_oracle(w20, tl, dim, policy, buffers):
     """Coset groups of up to 2^12 entries by dimension, on 2^11..2^13 tiles, with the fewest-bytes group
     boundaries (which put slots born or dying inside a group on coordinates above other pivots) and one
