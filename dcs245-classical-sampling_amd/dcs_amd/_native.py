@@ -19,7 +19,7 @@ MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED, MODE_COMPILED = 0, 1, 2, 3
 EXPORTED = (
     'dcs_last_error', 'dcs_get_device_count', 'dcs_get_device_info', 'dcs_context_create',
     'dcs_context_destroy', 'dcs_synchronize', 'dcs_plan_create', 'dcs_gather_plan_create', 'dcs_plan_destroy',
-    'dcs_plan_compile', 'dcs_plan_memory', 'dcs_plan_use_graphs',
+    'dcs_plan_compile', 'dcs_plan_kernel_source', 'dcs_plan_memory', 'dcs_plan_use_graphs',
     'dcs_plan_units',
     'dcs_plan_schedule', 'dcs_contract', 'dcs_block_zero', 'dcs_block_accumulate', 'dcs_block_transform',
     'dcs_block_read', 'dcs_block_device_pointer', 'dcs_draw', 'dcs_frontier_read', 'dcs_plan_profile',
@@ -84,6 +84,8 @@ def load():
         'dcs_gather_plan_create': (C.c_int, [vp, P(GatherDesc), P(vp)]),
         'dcs_plan_destroy': (None, [vp]),
         'dcs_plan_compile': (C.c_int, [vp, P(dbl)]),
+        'dcs_plan_kernel_source': (C.c_int, [P(PlanDesc), C.c_char_p, u64, P(u64), P(i32), P(i32), P(i32), P(i32),
+                                             C.c_int, P(C.c_int)]),
         'dcs_plan_memory': (C.c_int, [vp, P(u64)]),
         'dcs_plan_units': (C.c_int, [vp, C.c_int, P(C.c_int)]),
         'dcs_plan_schedule': (C.c_int, [vp, C.c_int, C.c_int, P(i32), P(i32), P(i32), P(i32), P(C.c_int)]),
@@ -170,6 +172,60 @@ class Context:
             pass
 
 
+def plan_desc(table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, phase, open_unitaries, blocks=1,
+              fuse_max_dim=6, resident_bits=12, tile_log2=11, exact_rounding=False, pass_max_dim=3, lds_storage=False,
+              group_policy=0, lds_buffers=2):
+    """dcs_plan_desc of an in-place step table and the arrays its pointers refer to (keep them alive)."""
+    n = len(table['kind'])
+    keep = {}
+
+    def arr(name, dt):
+        a = np.ascontiguousarray(table[name], dtype=dt)
+        keep[name] = a
+        return a
+    d = PlanDesc()
+    d.storage, d.maxslots, d.lanes, d.nsteps = int(storage), int(maxslots), int(lanes), n
+    d.kind = ptr(arr('kind', np.int32), C.c_int32)
+    d.K = ptr(arr('K', np.uint64), C.c_uint64)
+    d.Ka = ptr(arr('Ka', np.uint64), C.c_uint64)
+    d.pivot = ptr(arr('pivot', np.int32), C.c_int32)
+    d.pivot_a = ptr(arr('pivot_a', np.int32), C.c_int32)
+    d.beta = ptr(arr('beta', np.uint64), C.c_uint64)
+    d.in_mask = ptr(arr('in_mask', np.uint64), C.c_uint64)
+    d.out_mask = ptr(arr('out_mask', np.uint64), C.c_uint64)
+    keep['smap'] = np.ascontiguousarray(syndrome_map, dtype=np.uint64)
+    keep['phase'] = np.ascontiguousarray(phase, dtype=np.float64)
+    keep['U'] = np.ascontiguousarray(np.asarray(open_unitaries, dtype=np.complex128).reshape(-1, 4).view(np.float64))
+    d.k, d.terminal_mask = int(k), C.c_uint64(int(terminal_mask))
+    d.syndrome_map = ptr(keep['smap'], C.c_uint64)
+    d.phase = ptr(keep['phase'], C.c_double)
+    d.open_unitaries = ptr(keep['U'], C.c_double)
+    d.blocks, d.fuse_max_dim, d.resident_bits, d.tile_log2 = int(blocks), int(fuse_max_dim), int(resident_bits), int(tile_log2)
+    d.exact_rounding = int(bool(exact_rounding))
+    d.pass_max_dim = int(pass_max_dim)
+    d.lds_storage = int(bool(lds_storage))
+    d.group_policy, d.lds_buffers = int(group_policy), int(lds_buffers)
+    return d, keep
+
+
+def kernel_source(table, maxslots, storage, terminal_mask, k, **options):
+    """Generated group-kernel source of a step table and its group segments, without a device:
+    (source, [(kernel index, first step, step count, workgroup threads), ...]).  options: the
+    schedule options of Plan (fuse_max_dim, resident_bits, tile_log2, exact_rounding, pass_max_dim,
+    lds_storage, group_policy, lds_buffers)."""
+    lib = load()
+    kk = int(k)
+    d, keep = plan_desc(table, maxslots, storage, 1, kk, terminal_mask, np.zeros(1, np.uint64), np.zeros(1),
+                        np.tile(np.eye(2, dtype=np.complex128), (max(kk, 1), 1, 1)), **options)
+    n, length = C.c_int(), C.c_uint64()
+    check(lib.dcs_plan_kernel_source(C.byref(d), None, 0, C.byref(length), None, None, None, None, 0, C.byref(n)))
+    buf = C.create_string_buffer(length.value + 1)
+    seg = [np.zeros(max(1, n.value), np.int32) for _ in range(4)]
+    check(lib.dcs_plan_kernel_source(C.byref(d), buf, length.value + 1, C.byref(length),
+                                     *[ptr(a, C.c_int32) for a in seg], n.value, C.byref(n)))
+    return buf.value.decode(), [tuple(int(a[i]) for a in seg) for i in range(n.value)]
+
+
 class Plan:
     """Device-resident plan: frontier buffer(s), step tables, syndrome map, phase, block accumulators."""
 
@@ -178,34 +234,9 @@ class Plan:
                  pass_max_dim=3, lds_storage=False, group_policy=0, lds_buffers=2):
         self.ctx, self.lib = ctx, ctx.lib
         n = len(table['kind'])
-        keep = {}
-
-        def arr(name, dt):
-            a = np.ascontiguousarray(table[name], dtype=dt)
-            keep[name] = a
-            return a
-        d = PlanDesc()
-        d.storage, d.maxslots, d.lanes, d.nsteps = int(storage), int(maxslots), int(lanes), n
-        d.kind = ptr(arr('kind', np.int32), C.c_int32)
-        d.K = ptr(arr('K', np.uint64), C.c_uint64)
-        d.Ka = ptr(arr('Ka', np.uint64), C.c_uint64)
-        d.pivot = ptr(arr('pivot', np.int32), C.c_int32)
-        d.pivot_a = ptr(arr('pivot_a', np.int32), C.c_int32)
-        d.beta = ptr(arr('beta', np.uint64), C.c_uint64)
-        d.in_mask = ptr(arr('in_mask', np.uint64), C.c_uint64)
-        d.out_mask = ptr(arr('out_mask', np.uint64), C.c_uint64)
-        smap = np.ascontiguousarray(syndrome_map, dtype=np.uint64)
-        ph = np.ascontiguousarray(phase, dtype=np.float64)
-        U = np.ascontiguousarray(np.asarray(open_unitaries, dtype=np.complex128).reshape(-1, 4).view(np.float64))
-        d.k, d.terminal_mask = int(k), C.c_uint64(int(terminal_mask))
-        d.syndrome_map = ptr(smap, C.c_uint64)
-        d.phase = ptr(ph, C.c_double)
-        d.open_unitaries = ptr(U, C.c_double)
-        d.blocks, d.fuse_max_dim, d.resident_bits, d.tile_log2 = int(blocks), int(fuse_max_dim), int(resident_bits), int(tile_log2)
-        d.exact_rounding = int(bool(exact_rounding))
-        d.pass_max_dim = int(pass_max_dim)
-        d.lds_storage = int(bool(lds_storage))
-        d.group_policy, d.lds_buffers = int(group_policy), int(lds_buffers)
+        d, self._keep = plan_desc(table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, phase,
+                                  open_unitaries, blocks, fuse_max_dim, resident_bits, tile_log2, exact_rounding,
+                                  pass_max_dim, lds_storage, group_policy, lds_buffers)
         h = C.c_void_p()
         check(self.lib.dcs_plan_create(ctx.handle, C.byref(d), C.byref(h)))
         self.handle = h

@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -1253,17 +1254,24 @@ ModuleCache &module_cache() {
     return c;
 }
 
+// Source of the module holding one generated kernel per fused group of the plan's schedule.
+std::string group_module_source(const dcs_plan *p, const std::vector<GroupDev> &groups, const std::vector<PassDev> &passes,
+                                const std::vector<PStepDev> &psteps, const std::vector<u64> &addrv) {
+    CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0, p->lds_buffers};
+    std::string src = codegen::prelude(opt, kKernelsHpp);
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const GroupDev &g = groups[gi];
+        codegen::GroupView v{&g, passes.data() + g.first_pass, psteps.data(), addrv.data() + g.addrv_offset};
+        src += codegen::group_kernel(opt, (int)gi, v);
+    }
+    return src;
+}
+
 int compile_groups(dcs_plan *p, double *seconds) {
     if (seconds) *seconds = 0.0;
     if (p->executor != 0) return fail(DCS_ERR_INVALID, "a gather plan has no coset groups to compile");
     if (p->compiled) return DCS_OK;
-    CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0, p->lds_buffers};
-    std::string src = codegen::prelude(opt, kKernelsHpp);
-    for (size_t gi = 0; gi < p->h_groups.size(); ++gi) {
-        const GroupDev &g = p->h_groups[gi];
-        codegen::GroupView v{&g, p->h_passes.data() + g.first_pass, p->h_psteps.data(), p->h_addrv.data() + g.addrv_offset};
-        src += codegen::group_kernel(opt, (int)gi, v);
-    }
+    const std::string src = group_module_source(p, p->h_groups, p->h_passes, p->h_psteps, p->h_addrv);
     HIPCHK(hipSetDevice(p->ctx->device));
     hipModule_t mod = nullptr;
     {
@@ -1485,9 +1493,8 @@ void dcs_plan_destroy(dcs_plan *p) {
     delete p;
 }
 
-int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
-    if (!ctx || !d || !out) return fail(DCS_ERR_INVALID, "null argument");
-    *out = nullptr;
+// Checks of a plan description that need no device.
+int check_plan_desc(const dcs_plan_desc *d) {
     if (d->storage != DCS_STORAGE_C64 && d->storage != DCS_STORAGE_C128 && d->storage != DCS_STORAGE_F16X2)
         return fail(DCS_ERR_INVALID, "storage must be 0 (complex64), 1 (complex128) or 2 (binary16 pairs)");
     if (d->maxslots < 0 || d->maxslots > 40) return fail(DCS_ERR_INVALID, "maxslots %d out of range", d->maxslots);
@@ -1505,10 +1512,12 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     if (d->pass_max_dim < 3 || d->pass_max_dim > kPassMax) return fail(DCS_ERR_INVALID, "pass_max_dim must be 3 or 4");
     const u64 lim = (d->maxslots == 0) ? 1ull : ((1ull << d->maxslots) - 1);
     if (d->terminal_mask & ~lim) return fail(DCS_ERR_INVALID, "terminal slots beyond maxslots");
-    HIPCHK(hipSetDevice(ctx->device));
+    return DCS_OK;
+}
 
-    dcs_plan *p = new dcs_plan();
-    p->ctx = ctx;
+// Plan fields and validated host steps from a checked description (no device work).
+int ingest_plan_steps(const dcs_plan_desc *d, dcs_plan *p, std::vector<StepDev> &sdev) {
+    const u64 lim = (d->maxslots == 0) ? 1ull : ((1ull << d->maxslots) - 1);
     p->storage = d->storage;
     p->maxslots = d->maxslots;
     p->lanes = d->lanes;
@@ -1527,12 +1536,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     p->lane_stride = 1ull << d->maxslots;
     p->terminal_mask = d->terminal_mask;
     p->unitaries.assign(d->open_unitaries, d->open_unitaries + (size_t)d->k * 8);
-
-    auto bail = [&](int rc) {
-        dcs_plan_destroy(p);
-        return rc;
-    };
-    std::vector<StepDev> sdev(d->nsteps);
+    sdev.assign(d->nsteps, StepDev{});
     for (int s = 0; s < d->nsteps; ++s) {
         StepHost h;
         h.kind = d->kind[s];
@@ -1543,21 +1547,21 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
         h.beta = d->beta[s];
         h.in_mask = d->in_mask[s];
         h.out_mask = d->out_mask[s];
-        if (h.kind < 0 || h.kind > 7) return bail(fail(DCS_ERR_INVALID, "step %d: kind %d", s, h.kind));
+        if (h.kind < 0 || h.kind > 7) return (fail(DCS_ERR_INVALID, "step %d: kind %d", s, h.kind));
         if ((h.in_mask | h.out_mask | h.K | h.Ka | h.beta) & ~lim)
-            return bail(fail(DCS_ERR_INVALID, "step %d: mask beyond maxslots", s));
+            return (fail(DCS_ERR_INVALID, "step %d: mask beyond maxslots", s));
         const bool needK = h.kind == 3 || h.kind == 5 || h.kind == 6 || h.kind == 7;
         const bool needKa = h.kind == 1 || h.kind == 6;
         if (needK && (!h.K || top_bit(h.K) != h.pivot || (h.K & ~h.in_mask)))
-            return bail(fail(DCS_ERR_INVALID, "step %d: K / pivot inconsistent", s));
+            return (fail(DCS_ERR_INVALID, "step %d: K / pivot inconsistent", s));
         if (needKa && (!h.Ka || top_bit(h.Ka) != h.pivot_a || (h.Ka & ~h.in_mask)))
-            return bail(fail(DCS_ERR_INVALID, "step %d: Ka / pivot_a inconsistent", s));
-        if (h.kind == 6 && ((h.Ka >> h.pivot) & 1)) return bail(fail(DCS_ERR_INVALID, "step %d: Ka holds the K pivot", s));
+            return (fail(DCS_ERR_INVALID, "step %d: Ka / pivot_a inconsistent", s));
+        if (h.kind == 6 && ((h.Ka >> h.pivot) & 1)) return (fail(DCS_ERR_INVALID, "step %d: Ka holds the K pivot", s));
         if (h.kind == 0 && (h.pivot < 0 || h.pivot >= d->maxslots || ((h.in_mask >> h.pivot) & 1)))
-            return bail(fail(DCS_ERR_INVALID, "step %d: grow slot invalid", s));
+            return (fail(DCS_ERR_INVALID, "step %d: grow slot invalid", s));
         if (s > 0 && h.in_mask != p->steps.back().out_mask)
-            return bail(fail(DCS_ERR_INVALID, "step %d: live slots do not chain", s));
-        if (s == 0 && h.in_mask != 0) return bail(fail(DCS_ERR_INVALID, "first step must start from the empty basis"));
+            return (fail(DCS_ERR_INVALID, "step %d: live slots do not chain", s));
+        if (s == 0 && h.in_mask != 0) return (fail(DCS_ERR_INVALID, "first step must start from the empty basis"));
         h.slots = orbit_slots(h);
         p->steps.push_back(h);
         StepArgs a;
@@ -1583,7 +1587,30 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
         sd.nslots = popc(h.slots);
     }
     if (p->steps.back().out_mask != d->terminal_mask)
-        return bail(fail(DCS_ERR_INVALID, "terminal slots differ from the live slots after the last step"));
+        return (fail(DCS_ERR_INVALID, "terminal slots differ from the live slots after the last step"));
+    return DCS_OK;
+}
+
+int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
+    if (!ctx || !d || !out) return fail(DCS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    {
+        const int rc = check_plan_desc(d);
+        if (rc) return rc;
+    }
+    HIPCHK(hipSetDevice(ctx->device));
+
+    dcs_plan *p = new dcs_plan();
+    p->ctx = ctx;
+    auto bail = [&](int rc) {
+        dcs_plan_destroy(p);
+        return rc;
+    };
+    std::vector<StepDev> sdev;
+    {
+        const int rc = ingest_plan_steps(d, p, sdev);
+        if (rc) return bail(rc);
+    }
 
     std::vector<GroupDev> groups;
     std::vector<PassDev> passes;
@@ -1777,6 +1804,46 @@ int dcs_gather_plan_create(dcs_context *ctx, const dcs_gather_desc *d, dcs_plan 
 int dcs_plan_compile(dcs_plan *p, double *seconds) {
     if (!p) return fail(DCS_ERR_INVALID, "null plan");
     return compile_groups(p, seconds);
+}
+
+int dcs_plan_kernel_source(const dcs_plan_desc *d, char *src, uint64_t src_cap, uint64_t *src_len, int32_t *group,
+                           int32_t *first, int32_t *count, int32_t *threads, int cap, int *ngroups) {
+    if (!d || !src_len || !ngroups) return fail(DCS_ERR_INVALID, "null argument");
+    {
+        const int rc = check_plan_desc(d);
+        if (rc) return rc;
+    }
+    std::unique_ptr<dcs_plan> p(new dcs_plan());  // no context: nothing here touches a device
+    std::vector<StepDev> sdev;
+    std::vector<GroupDev> groups;
+    std::vector<PassDev> passes;
+    std::vector<PStepDev> psteps;
+    std::vector<u64> addrv;
+    {
+        int rc = ingest_plan_steps(d, p.get(), sdev);
+        if (!rc) rc = build_schedules(p.get(), groups, passes, psteps, addrv);
+        if (rc) return rc;
+    }
+    const std::string text = group_module_source(p.get(), groups, passes, psteps, addrv);
+    *src_len = text.size();
+    if (src && src_cap) {
+        const size_t n = std::min<size_t>(text.size(), (size_t)src_cap - 1);
+        memcpy(src, text.data(), n);
+        src[n] = '\0';
+    }
+    int n = 0;
+    for (const Seg &sg : p->sched_fused) {
+        if (sg.type != 2) continue;
+        if (n < cap) {
+            if (group) group[n] = sg.group;
+            if (first) first[n] = sg.first;
+            if (count) count[n] = sg.count;
+            if (threads) threads[n] = codegen::group_threads(groups[sg.group]);
+        }
+        ++n;
+    }
+    *ngroups = n;
+    return DCS_OK;
 }
 
 int dcs_plan_memory(dcs_plan *p, uint64_t *bytes) {

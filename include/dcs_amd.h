@@ -25,6 +25,8 @@
  *                             .alloc (canonical gather executor)       sampling/conditional_block_gpu.py:71-90
  *   dcs_contract on a gather  ConditionalBlockGPU._prefix (one launch   sampling/conditional_block_gpu.py:92-126
  *   plan                      of scan_step_scaled per step)            dcs_rankwidth/gpu_kernel.py:58-101
+ * dcs_plan_kernel_source has no counterpart in the reference: it returns the generated group kernels
+ * of a plan without a device, for inspection and for emulating them on the CPU.
  */
 #ifndef DCS_AMD_H
 #define DCS_AMD_H
@@ -139,6 +141,14 @@ void dcs_plan_destroy(dcs_plan *plan);
  * *seconds (may be NULL) receives the compile time (0 when served from the cache).  dcs_contract in
  * DCS_MODE_COMPILED calls it on first use. */
 int dcs_plan_compile(dcs_plan *plan, double *seconds);
+/* Source of the group kernels that dcs_plan_compile would build for this description, and the fused
+ * group segments of the compiled schedule, without a device.  src (capacity src_cap, may be NULL)
+ * receives the NUL-terminated source, cut to fit; *src_len its full length.  For each group segment i
+ * < cap: group[i] the kernel index (dcs_group_<index>), first[i] and count[i] its steps, threads[i]
+ * its workgroup size; *ngroups the number of segments.  The output arrays may be NULL. */
+int dcs_plan_kernel_source(const dcs_plan_desc *desc, char *src, uint64_t src_cap, uint64_t *src_len,
+                           int32_t *group, int32_t *first, int32_t *count, int32_t *threads, int cap,
+                           int *ngroups);
 /* Canonical gather executor: two tables of 2^rmax entries; step s maps the 2^r_in table to the
  * 2^r_out table, out[c'] = sum over the step's kernel combinations of E[p & (2^r_in - 1)] * (+-f_y),
  * p = XOR_b tables[b][byte b of c'] ^ combination.  dcs_contract on such a plan accepts

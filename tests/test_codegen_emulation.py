@@ -1,0 +1,84 @@
+"""Generated coset-group kernels emulated on the CPU (no GPU).
+
+dcs_plan_kernel_source returns, without a device, the exact source the library compiles with hipRTC
+for a plan. This test compiles that source with the host compiler against tests/emu/shim.hpp (one
+thread per work item, a barrier per workgroup, synchronous global-to-LDS loads) and runs every group
+kernel of the width-20 synthetic plan. With per-step rounding (or complex128 storage, where rounding is
+the identity) each kernel's live entries and recorded maximum must equal the per-step orbit formulas
+bit for bit, and so must the whole chain of groups from |0...0> with the scale carried between them.
+
+It pins what the device parity tests can only see through their results: the coset addressing, the
+register passes, dead-entry handling, the LDS index permutation and deferred stores of the
+asynchronous-load pipeline, and the one- and two-tile pipelines. It does not pin the asynchronous
+timing, which the emulator runs synchronously; the ordering argument for that is in the generator's
+comments.
+"""
+import re
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from dcs_amd import _native
+from dcs_amd.block_plan import BlockPlan
+from dcs_amd.stabilizer import synthetic_instance
+
+EMU = Path(__file__).resolve().parent / 'emu'
+
+# (storage, tile_log2, fuse_max_dim, group_policy, lds_buffers, exact_rounding, gridcap)
+CASES = [
+    (0, 12, 8, 1, 3, True, 1),    # complex64, asynchronous loads, one workgroup walks every tile
+    (0, 11, 6, 0, 2, True, 3),    # complex64, two register-fed LDS tiles on 2^11 tiles, greedy groups
+    (1, 12, 8, 1, 1, False, 2),   # complex128, one LDS tile
+    (2, 12, 8, 2, 3, True, 2),    # binary16 storage (one-tile pipeline), bytes-and-passes groups
+]
+
+
+def adapt(src):
+    """The generated source as the host compiler takes it: no fixed-width typedefs (the shim's headers
+    have them), no GPU attributes, the LDS-only barrier as a workgroup barrier."""
+    src = '\n'.join(line for line in src.split('\n') if not re.match(r'typedef .*int.*_t;', line))
+    src = src.replace('__attribute__((amdgpu_waves_per_eu(4)))', '')
+    src = re.sub(r'__attribute__\(\(address_space\(\d\)\)\)', '', src)
+    src, n = re.subn(r'\{ asm volatile\(.*?s_barrier.*?\); \}', '{ __syncthreads(); }', src)
+    assert n == 1, 'expected one LDS barrier definition in the prelude'
+    return src
+
+
+@pytest.fixture(scope='module')
+def w20():
+    inst = synthetic_instance(28, 32, 64, seed=0)
+    return BlockPlan(inst['graph'], inst['unitaries'], 28, inst['natural_order'], [], list(range(20, 28)))
+
+
+@pytest.mark.skipif(shutil.which('g++') is None, reason='needs the host C++ compiler')
+@pytest.mark.parametrize('storage,tl,dim,policy,buffers,exact,gridcap', CASES)
+def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, policy, buffers, exact, gridcap):
+    t = w20.step_table()
+    src, segs = _native.kernel_source(t, w20.inplace_maxslots, storage, w20.terminal_mask(), w20.k,
+                                      fuse_max_dim=dim, resident_bits=0, tile_log2=tl, exact_rounding=exact,
+                                      pass_max_dim=3, lds_storage=True, group_policy=policy, lds_buffers=buffers)
+    assert segs and len(set(s[0] for s in segs)) == len(segs)
+    if buffers == 3 and storage == 0:
+        assert 'global_load_lds' in src        # the asynchronous pipeline is what runs here
+    (tmp_path / 'kernels.inc').write_text(adapt(src))
+    names = sorted({int(m) for m in re.findall(r'\bdcs_group_(\d+)\(St ', src)})
+    assert names == list(range(len(names)))
+    (tmp_path / 'table.inc').write_text(''.join(f'&dcs_group_{i},\n' for i in names))
+    rows = [f"{len(t['kind'])} {w20.inplace_maxslots} {w20.terminal_mask()}"]
+    for i in range(len(t['kind'])):
+        rows.append(' '.join(str(int(t[k][i])) for k in ('kind', 'K', 'Ka', 'pivot', 'pivot_a', 'beta', 'in_mask',
+                                                         'out_mask')))
+    (tmp_path / 'plan.txt').write_text('\n'.join(rows) + '\n')
+    (tmp_path / 'groups.txt').write_text(''.join(f'{g} {a} {c} {th}\n' for g, a, c, th in segs))
+    exe = tmp_path / 'emu'
+    subprocess.run(['g++', '-std=c++20', '-O1', '-pthread', '-w', f'-I{EMU}', f'-I{tmp_path}', str(EMU / 'emu.cpp'),
+                    '-o', str(exe)], check=True, timeout=600)
+    r = subprocess.run([str(exe), str(tmp_path / 'plan.txt'), str(tmp_path / 'groups.txt'), str(gridcap)],
+                       capture_output=True, text=True, timeout=900)
+    print(r.stdout[-3000:])
+    lines = r.stdout.splitlines()
+    assert sum(line.startswith('group ') for line in lines) == len(segs)
+    assert sum(line.startswith('chain unit ') for line in lines) == len(segs)  # resident_bits=0: groups cover all
+    assert r.returncode == 0 and lines[-1] == 'RESULT ok (0 bad)', r.stdout[-2000:]
