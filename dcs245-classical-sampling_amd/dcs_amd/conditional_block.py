@@ -246,7 +246,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
     _c64_storage = _native.STORAGE_C64   # frontier storage behind dtype=np.complex64
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
-              fuse_max_dim=8, resident_bits=12, tile_log2=12, blocks=1, exact_rounding=False,
+              fuse_max_dim=None, resident_bits=12, tile_log2=None, blocks=1, exact_rounding=False,
               pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=3, graphs=False):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
 
@@ -263,7 +263,8 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    to storage once per register pass
         fuse_max_dim, tile_log2, pass_max_dim  largest coset dimension of a fused group, log2 entries
                    of its LDS tile (11, 12; 13 compiled only), span of one register pass (3, 4); the
-                   defaults (8, 12, 3) are the fastest measured on the 64-qubit plans
+                   defaults are the fastest measured on the 64-qubit plans: (8, 12, 3), and (9, 13, 3)
+                   for complex128 in the compiled schedule when neither of the first two is given
         group_policy  0: greedy maximal coset groups; 1 (default): the group boundaries with the fewest
                    HBM bytes (dynamic programming); 2: bytes and register-pass work summed; 3: the
                    same two costs overlapped, max plus a quarter of the smaller (both calibrated on
@@ -287,6 +288,10 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
         if bp.terminal_rank > bp.k:
             raise ValueError(f'terminal rank {bp.terminal_rank} exceeds the {bp.k} open vertices')
         self.mode = SCHEDULES[schedule]
+        if fuse_max_dim is None and tile_log2 is None and self.mode == _native.MODE_COMPILED and self.use_fp64:
+            fuse_max_dim, tile_log2 = 9, 13
+        fuse_max_dim = 8 if fuse_max_dim is None else fuse_max_dim
+        tile_log2 = 12 if tile_log2 is None else tile_log2
         self.lanes = int(lanes)
         self.ctx = _native.Context(device)
         storage = _native.STORAGE_C128 if self.use_fp64 else self._c64_storage
