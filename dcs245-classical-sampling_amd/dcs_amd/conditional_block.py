@@ -340,10 +340,13 @@ class HalfConditionalBlockGPU(InplaceConditionalBlockGPU):
 
     _c64_storage = _native.STORAGE_F16X2
 
-    def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, **kw):
+    def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, group_policy=2, **kw):
+        """As InplaceConditionalBlockGPU.alloc; group boundaries default to policy 2 (bytes and register
+        passes), which measured fastest with 4-byte entries on the 64-qubit half_k1 plan."""
         if np.dtype(dtype) != np.dtype(np.complex64):
             raise ValueError('Half storage exports complex64; use baseline executor for complex128')
-        super().alloc(dtype=np.complex64, device=device, fp64_kernel_path=fp64_kernel_path, **kw)
+        super().alloc(dtype=np.complex64, device=device, fp64_kernel_path=fp64_kernel_path, group_policy=group_policy,
+                      **kw)
         self.inplace_storage = 'float16-components'
 
     def block(self, *args, **kwargs):
