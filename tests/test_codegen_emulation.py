@@ -82,3 +82,28 @@ def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, p
     assert sum(line.startswith('group ') for line in lines) == len(segs)
     assert sum(line.startswith('chain unit ') for line in lines) == len(segs)  # resident_bits=0: groups cover all
     assert r.returncode == 0 and lines[-1] == 'RESULT ok (0 bad)', r.stdout[-2000:]
+
+
+def test_kernel_source_modes_and_errors(w20):
+    """Asynchronous global-to-LDS loads are generated for complex64 tiles only (complex128 and binary16
+    groups take the one-tile pipeline under lds_buffers=3); a description the library rejects raises
+    without touching a device."""
+    t = w20.step_table()
+    kw = dict(fuse_max_dim=8, resident_bits=12, tile_log2=12, exact_rounding=False, pass_max_dim=3,
+              lds_storage=True, group_policy=1, lds_buffers=3)
+    args = (t, w20.inplace_maxslots)
+    c64, segs64 = _native.kernel_source(*args, 0, w20.terminal_mask(), w20.k, **kw)
+    c128, segs128 = _native.kernel_source(*args, 1, w20.terminal_mask(), w20.k, **kw)
+    h16, segs16 = _native.kernel_source(*args, 2, w20.terminal_mask(), w20.k, **kw)
+    assert 'global_load_lds(' in c64 and 'global_load_lds(' not in c128 and 'global_load_lds(' not in h16
+    assert segs64 == segs128 == segs16          # the grouping does not depend on the storage
+    assert [s[1] for s in segs64] == sorted(s[1] for s in segs64)
+    with pytest.raises(ValueError, match='group_policy'):
+        _native.kernel_source(*args, 0, w20.terminal_mask(), w20.k, **{**kw, 'group_policy': 7})
+    with pytest.raises(ValueError, match='tile_log2'):
+        _native.kernel_source(*args, 0, w20.terminal_mask(), w20.k, **{**kw, 'tile_log2': 14})
+    bad = dict(t)
+    bad['in_mask'] = t['in_mask'].copy()
+    bad['in_mask'][3] ^= 1 << 40                # a live slot beyond maxslots
+    with pytest.raises(ValueError):
+        _native.kernel_source(bad, w20.inplace_maxslots, 0, w20.terminal_mask(), w20.k, **kw)
