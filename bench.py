@@ -8,15 +8,16 @@ device CDF draw.  Inputs (graph, plan, frontier buffer) are resident before the 
 
 Fast path (value): complex64 storage, plan inplace_k2_fp32 (4 slices, rmax 34, 128 GiB frontier per
 lane; two prefixes at a time by default, their slices sharing each launch),
-compiled schedule (LDS-resident runs + one generated kernel per coset group, complex64 LDS tiles;
-coset groups of dimension <= 8 on 2^12-entry tiles with the fewest-HBM-bytes group boundaries).
+compiled schedule (LDS-resident runs + one generated kernel per coset group, complex64 LDS tiles
+filled by global-to-LDS loads; coset groups of dimension <= 8 on 2^12-entry tiles with the
+fewest-HBM-bytes group boundaries).
 Plain path (reported beside it): complex128
 storage, plan inplace_k3_fp64 (8 slices, rmax 33), one launch per step and a host read of every
 maximum, one prefix at a time -- the reference method.
 
 The line also carries the roofline of the dominant kernels (the coset-group kernels, HBM-bound)
 measured with HIP events on the library's stream, and a CPU baseline: the C oracle (OpenMP) on a bounded sample of the
-same workload (the first steps of one slice, widths <= 2^28), extrapolated by bytes.
+same workload (the first steps of one slice, live widths <= 2^32), extrapolated by bytes.
 """
 import argparse
 import json
