@@ -201,6 +201,31 @@ def test_prefix_blocks_and_lanes():
         e3.close()
 
 
+def test_w20_batched_lanes_equal_alone(w20):
+    """Width-20 plan with the default schedule (asynchronous tile loads): two prefixes batched into the
+    same group-kernel launches (lanes=2, one lane each) give the blocks each prefix gives alone with
+    lanes=1, bit for bit, and the same device draws."""
+    print('case w20 batched lanes', flush=True)
+    rng = np.random.default_rng(23)
+    n = w20['args'][2]
+    xs = [w20['x'], rng.integers(0, 2, n, dtype=np.uint8)]
+    us = (0.013, 0.37, 0.91)
+    e1 = engine(w20['args'], dtype=np.complex64)
+    alone, draws = [], []
+    for x in xs:
+        v = e1.block(x)[0]
+        alone.append(np.asarray(v))
+        draws.append([v.draw(u)['A_index'] for u in us])
+    e1.close()
+    e2 = engine(w20['args'], dtype=np.complex64, lanes=2, blocks=2)
+    vecs, meta = e2.block_many(xs)
+    for v, a, d in zip(vecs, alone, draws):
+        assert np.array_equal(np.asarray(v), a)
+        assert [v.draw(u)['A_index'] for u in us] == d
+    e2.close()
+    assert rel(alone[0], w20['ref128']) < 1e-5
+
+
 def test_parity_sliced_ring_circuit():
     """Parity slices on reference vertices (Y, X and Z operations) against the unsliced oracle."""
     print('case parity slices', flush=True)
