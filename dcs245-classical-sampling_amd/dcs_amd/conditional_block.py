@@ -1,7 +1,7 @@
 """Coherent, parity-sliced conditional output blocks on one MI355X.
 
 Drop-in for the reference engine classes (sampling/conditional_block_gpu.py:24-177,
-sampling/conditional_block_inplace_gpu.py:17-38, kernel_opt/inplace_gpu.py:219-263):
+sampling/conditional_block_inplace_gpu.py:17-38, kernel_opt/inplace_gpu.py:84-128):
 
     engine = InplaceConditionalBlockGPU(graph, unitaries, n, order, parity_operations, open_vertices)
     engine.alloc(dtype=np.complex64)            # or np.complex128 for audit blocks
@@ -32,7 +32,7 @@ SCHEDULES = {'steps': _native.MODE_STEPS, 'steps_host': _native.MODE_STEPS_HOST,
 
 def logscale_from_maxima(m2):
     """Replay the reference's host scale chain over the unit maxima of one slice
-    (kernel_opt/inplace_gpu.py:243-258): before unit u, e = frexp(sqrt(previous max))[1] and
+    (kernel_opt/inplace_gpu.py:108-123): before unit u, e = frexp(sqrt(previous max))[1] and
     ls += e ln 2.  Returns (ls, alive); alive is False when a unit's maximum is exactly zero
     (slice skipped).  A nonfinite maximum raises ArithmeticError."""
     ls = 0.
@@ -241,14 +241,14 @@ class ConditionalBlockGPU:
 
 
 class InplaceConditionalBlockGPU(ConditionalBlockGPU):
-    """Reference-compatible in-place engine (conditional_block_inplace_gpu.py:17-38, 194-215)."""
+    """Reference-compatible in-place engine (conditional_block_inplace_gpu.py:17-38)."""
 
     _c64_storage = _native.STORAGE_C64   # frontier storage behind dtype=np.complex64
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
               fuse_max_dim=None, resident_bits=12, tile_log2=None, blocks=1, exact_rounding=False,
               pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=3, graphs=False):
-        """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:220-240).
+        """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:85-105).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
         lanes      frontiers held at once; slices of one block are contracted `lanes` at a time
@@ -313,7 +313,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
 
     # ---- reference mixin hooks -------------------------------------------------------------
     def prefix_inplace(self, f, yfix, lane=0):
-        """(env or None, logscale, offset) of one slice (kernel_opt/inplace_gpu.py:242-259).
+        """(env or None, logscale, offset) of one slice (kernel_opt/inplace_gpu.py:107-124).
         env is the host copy of the terminal environment in storage precision."""
         coeffs, ov, ys, offset = self.bp.step_parameters(f, yfix)
         m2 = self._plan.contract(self.mode, [lane], coeffs, ov, ys)[0]

@@ -3,7 +3,7 @@
 //
 // Kernels
 //   k_step      one in-place step over the whole frontier, one thread per orbit; fused
-//               power-of-two scale and max |out|^2 (kernel_opt/inplace_gpu.py:155-198).
+//               power-of-two scale and max |out|^2 (kernel_opt/inplace_gpu.py:20-63).
 //   k_resident  a run of consecutive narrow steps whose live slots all sit below 2^RB: the frontier
 //               stays in LDS, one workgroup per lane, the exact per-step scale chain.
 //   k_coset     a fused group of consecutive steps.  The XOR masks of the group span a subspace V
@@ -1388,7 +1388,7 @@ int run_contract(dcs_plan *p, int mode, int nl, std::vector<u64> &ntiles_of_grou
         else rc = launch_coset<S>(p, nl, sg, nunits, ntiles_of_group[sg.group]);
         if (rc) return rc;
         if (mode == DCS_MODE_STEPS_HOST) {
-            // The reference driver reads the maximum back after every step (inplace_gpu.py:255-258).
+            // The reference driver reads the maximum back after every step (inplace_gpu.py:120-123).
             HIPCHK(hipMemcpyAsync(p->h_dmax, p->d_dmax, sizeof(u64) * (size_t)nl * nunits, hipMemcpyDeviceToHost,
                                   p->ctx->stream));
             HIPCHK(hipStreamSynchronize(p->ctx->stream));

@@ -85,7 +85,7 @@ __device__ __forceinline__ double max_nan(double a, double b) {
 }
 
 // Scale of the next unit from the previous maximum |out|^2:  2^-e with e = frexp(sqrt(m2)).exp,
-// as the reference host driver computes it (kernel_opt/inplace_gpu.py:250,258).  With m2 = M 2^E2,
+// as the reference host driver computes it (kernel_opt/inplace_gpu.py:115,123).  With m2 = M 2^E2,
 // M in [0.5, 1): sqrt(m2) has frexp exponent ceil(E2 / 2), exactly, for a correctly rounded sqrt.
 __device__ __forceinline__ double scale_from_m2(double m2) {
     int E2 = 0;
@@ -138,7 +138,7 @@ __device__ __forceinline__ u64 insert0_64(u64 x, int p) {
     return ((x ^ lo) << 1) | lo;
 }
 
-// ---- the eight orbit updates (kernel_opt/inplace_plan.py:97-127, inplace_gpu.py:163-194) --------
+// ---- the eight orbit updates (kernel_opt/inplace_plan.py:97-127, inplace_gpu.py:28-59) --------
 // sc, sd, sc1, sd1: sign bits at c, d = c^Ka, c1 = c^K, d1 = d^K.
 __device__ __forceinline__ void orb_grow(cd a, cd f0, cd f1, int sc, cd &o0, cd &o1) {
     o0 = cmul(f0, a);

@@ -5,7 +5,7 @@ fixes a parity of w and its neighbours: a local complementation at w (Y) or a pi
 (w, u) (X), followed by the removal of w.  With
     |G*v> = exp(+i pi/4 X_v) prod_{u in N(v)} exp(-i pi/4 Z_u) |G>   (up to a phase)
 the local Cliffords update as U_v <- U_v exp(-i pi/4 X), U_u <- U_u exp(+i pi/4 Z)
-(dcs_rankwidth/parity_slicing.py:3-11,158-173).
+(dcs_rankwidth/parity_slicing.py:3-11,23-32).
 """
 from __future__ import annotations
 
@@ -22,7 +22,7 @@ RZ_PLUS = np.array([[_C + 1j * _S, 0], [0, _C - 1j * _S]], dtype=complex)
 
 def lc(A, U, v):
     """Local complementation at v, in place, on adjacency A (uint8 N x N) and the local Cliffords
-    U (dict or list of 2x2) (parity_slicing.py:164-173)."""
+    U (dict or list of 2x2) (parity_slicing.py:23-32)."""
     nb = np.flatnonzero(A[v])
     if len(nb) > 1:
         A[np.ix_(nb, nb)] ^= 1
@@ -35,7 +35,7 @@ def lc(A, U, v):
 
 def apply_parity_operations(A, U, operations):
     """Apply [(kind, w, u)] in order: 'Y' = lc(w); 'X' = pivot lc(u) lc(w) lc(u); 'Z' = nothing
-    (conditional_block_gpu.py:35-41; parity_slicing.py:256-260)."""
+    (conditional_block_gpu.py:35-41; parity_slicing.py:115-119)."""
     for kind, w, u in operations:
         if kind == 'Y':
             lc(A, U, int(w))

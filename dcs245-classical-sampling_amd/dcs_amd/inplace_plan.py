@@ -13,7 +13,7 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
-# Orbit codes shared with the HIP kernels and the C oracle (kernel_opt/inplace_gpu.py:209-217).
+# Orbit codes shared with the HIP kernels and the C oracle (kernel_opt/inplace_gpu.py:74-82).
 KIND_GROW, KIND_MIX_PAIR, KIND_MIX_DIAG, KIND_REPLACE = 0, 1, 2, 3
 KIND_FIXED, KIND_FIXED_SHRINK, KIND_SHRINK_QUAD, KIND_SHRINK_PAIR = 4, 5, 6, 7
 
@@ -130,7 +130,7 @@ def compile_inplace(order, future, fixed, stop=None):
 
 
 def orbit_descriptor(st):
-    """(orbit code, omitted slots) of a step (inplace_gpu.py:209-217).
+    """(orbit code, omitted slots) of a step (inplace_gpu.py:74-82).
 
     The omitted slots are the coordinates that are fixed to zero in the orbit representative."""
     kind = st.kind
@@ -150,6 +150,6 @@ def orbit_descriptor(st):
 
 
 def orbit_slots(st):
-    """Physical slots enumerated by the orbit index, ascending (inplace_gpu.py:233)."""
+    """Physical slots enumerated by the orbit index, ascending (inplace_gpu.py:98)."""
     _, omit = orbit_descriptor(st)
     return [p for p in st.active_in if p not in omit]

@@ -217,7 +217,7 @@ _EXACT = np.array([0.0, 0.5, -0.5, 1.0, -1.0, 2 ** -0.5, -(2 ** -0.5)])
 
 def snap(M):
     """Project each entry onto the exact Clifford values {0, +-1/2, +-1, +-1/sqrt 2}
-    (t_injection.py:316-323)."""
+    (t_injection.py:52-59)."""
     M = np.asarray(M, dtype=np.complex128)
 
     def f(x):

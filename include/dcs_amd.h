@@ -8,19 +8,19 @@
  * last failure on the calling thread is available from dcs_last_error().
  *
  * Reference interfaces replaced (paths relative to the reference's source/ directory):
- *   dcs_context_create        cp.cuda.Device(device).use()            kernel_opt/inplace_gpu.py:221-222
+ *   dcs_context_create        cp.cuda.Device(device).use()            kernel_opt/inplace_gpu.py:86-87
  *   dcs_get_device_info       getDeviceProperties / runtime versions  production/worker.py:60-76
- *   dcs_plan_create           InplaceGPUMixin.alloc (plan upload,     kernel_opt/inplace_gpu.py:220-240
- *                             frontier, syndrome map, phase)          sampling/conditional_block_inplace_gpu.py:195-211
- *   dcs_contract              InplaceGPUMixin.prefix_inplace           kernel_opt/inplace_gpu.py:242-259
- *                             (one launch of inplace_step per step)   kernel_opt/inplace_gpu.py:155-198
+ *   dcs_plan_create           InplaceGPUMixin.alloc (plan upload,     kernel_opt/inplace_gpu.py:85-105
+ *                             frontier, syndrome map, phase)          sampling/conditional_block_inplace_gpu.py:18-34
+ *   dcs_contract              InplaceGPUMixin.prefix_inplace           kernel_opt/inplace_gpu.py:107-124
+ *                             (one launch of inplace_step per step)   kernel_opt/inplace_gpu.py:20-63
  *   dcs_block_zero,
  *   dcs_block_accumulate      total = zeros; total[indices] += env*coef  sampling/conditional_block_gpu.py:137,155-158
  *   dcs_block_transform       Walsh-Hadamard, phase, per-qubit U      sampling/conditional_block_gpu.py:159-171
  *   dcs_block_read            cp.asnumpy(vector)                      production/worker.py:109
  *   dcs_draw                  draw_checked primary CDF (cumsum,       sampling/sampling_protocol.py:56-65,81-82
  *                             searchsorted side='right')
- *   dcs_frontier_read         d_inplace[d_inplace_terminal_indices]   kernel_opt/inplace_gpu.py:237-239,259
+ *   dcs_frontier_read         d_inplace[d_inplace_terminal_indices]   kernel_opt/inplace_gpu.py:102-104,124
  *   dcs_gather_plan_create    GPUMixin.alloc + ConditionalBlockGPU     dcs_rankwidth/gpu_kernel.py:106-121
  *                             .alloc (canonical gather executor)       sampling/conditional_block_gpu.py:71-90
  *   dcs_contract on a gather  ConditionalBlockGPU._prefix (one launch   sampling/conditional_block_gpu.py:92-126
@@ -199,7 +199,7 @@ int dcs_contract(dcs_plan *plan, int mode, int nlanes, const int32_t *lanes, con
  * schedule, maxima read-back) into a HIP graph once per (mode, nlanes) and replays it on later calls;
  * 0 (default): launches every call, and frees the captured graphs.  DCS_MODE_STEPS_HOST and profiling
  * always launch directly.  No counterpart in the reference (CuPy launches every step from Python,
- * kernel_opt/inplace_gpu.py:246-259); it serves callers that contract many prefixes of one plan. */
+ * kernel_opt/inplace_gpu.py:111-124); it serves callers that contract many prefixes of one plan. */
 int dcs_plan_use_graphs(dcs_plan *plan, int enable);
 
 int dcs_block_zero(dcs_plan *plan, int block);

@@ -5,7 +5,7 @@
  * It restates, in plain C with OpenMP over orbits:
  *   oracle_contract        kernel_opt/inplace_plan.py:90-135 (execute_inplace, the eight orbit
  *                          updates) with the scale chain of the GPU driver
- *                          kernel_opt/inplace_gpu.py:242-258 (step j multiplied by 2^-e,
+ *                          kernel_opt/inplace_gpu.py:107-123 (step j multiplied by 2^-e,
  *                          e = frexp(sqrt(max |out_{j-1}|^2)).exp; max of the stored values); storage
  *                          complex64, complex128 or binary16 pairs (kernel_opt/inplace_half_gpu.py:14-78)
  *   oracle_f32_to_h16,     the binary16 conversions of that storage

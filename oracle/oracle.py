@@ -6,7 +6,7 @@ as the checker (or, for cpu_baseline, as the timed CPU port).  The product path 
 What each function restates and what pins it:
 
   contract_inplace   C restatement (liboracle.so) of execute_inplace (kernel_opt/inplace_plan.py:90-135)
-                     with the GPU driver's scale chain (kernel_opt/inplace_gpu.py:242-258); storage 2 is
+                     with the GPU driver's scale chain (kernel_opt/inplace_gpu.py:107-123); storage 2 is
                      the binary16-pair storage of kernel_opt/inplace_half_gpu.py:14-78, whose
                      conversions (f32_to_h16, h16_to_f32) are pinned against numpy's float16 over every
                      half and over float32 boundary and random patterns.  The binary16 contraction is
@@ -82,7 +82,7 @@ def _p(a, t):
 
 
 def frexp_scale_chain(m2):
-    """ls and liveness of one slice from its per-step maxima (inplace_gpu.py:250-258)."""
+    """ls and liveness of one slice from its per-step maxima (inplace_gpu.py:115-123)."""
     ls, maximum = 0., 1.
     for m in m2:
         ls += math.frexp(maximum)[1] * math.log(2.)
