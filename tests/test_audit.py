@@ -76,3 +76,15 @@ def test_masses_certificate_and_input_checks():
         audit_vector_pair(np.ones(4), np.zeros(4), 0)
     with pytest.raises(ValueError):
         certificate_from_losses([1.2], 1.0)
+
+
+def test_bound_is_the_smallest_float_reaching_the_threshold():
+    """b* is where n KL(mean || b) first reaches log(1 / alpha): the float below it falls short, and it
+    agrees with an independent root finder (scipy's Brent method on the same function) to 1e-14."""
+    from scipy.optimize import brentq
+    for mean, n, alpha in [(1e-4, 64, 0.05), (0.0216, 64, 0.05), (0.2, 7, 0.01), (0.9, 1000, 0.1)]:
+        t = math.log(1 / alpha) / n
+        b = bounded_mean_upper(mean, n, alpha)
+        assert kl_bernoulli(mean, b) >= t > kl_bernoulli(mean, math.nextafter(b, 0.))
+        root = brentq(lambda x: kl_bernoulli(mean, x) - t, mean, 1 - 1e-300, xtol=1e-16, rtol=1e-15)
+        assert abs(b - root) <= 1e-14
