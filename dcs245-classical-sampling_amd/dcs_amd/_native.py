@@ -13,6 +13,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / 'libdcs_amd.so'
 
 DCS_OK, DCS_ERR_INVALID, DCS_ERR_RUNTIME, DCS_ERR_NOMEM, DCS_ERR_NONFINITE, DCS_ERR_ZERO = range(6)
+DRAW_NOALLOC_MAX_K = 22   # DCS_DRAW_NOALLOC_MAX_K
 STORAGE_C64, STORAGE_C128, STORAGE_F16X2 = 0, 1, 2
 MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED, MODE_COMPILED = 0, 1, 2, 3
 
@@ -23,7 +24,7 @@ EXPORTED = (
     'dcs_plan_units',
     'dcs_plan_schedule', 'dcs_contract', 'dcs_block_zero', 'dcs_block_accumulate', 'dcs_block_transform',
     'dcs_block_read', 'dcs_block_device_pointer', 'dcs_draw', 'dcs_frontier_read', 'dcs_plan_profile',
-    'dcs_stream_copy')
+    'dcs_stream_copy', 'dcs_draw_noalloc')
 
 
 class DeviceInfo(C.Structure):
@@ -100,6 +101,7 @@ def load():
         'dcs_plan_profile': (C.c_int, [vp, C.c_int, P(dbl), P(C.c_int64)]),
         'dcs_plan_use_graphs': (C.c_int, [vp, C.c_int]),
         'dcs_stream_copy': (C.c_int, [vp, u64, C.c_int, P(dbl)]),
+        'dcs_draw_noalloc': (C.c_int, [vp, vp, C.c_int, dbl, P(DrawResult)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -121,6 +123,12 @@ def check(rc):
     if rc in (DCS_ERR_NONFINITE, DCS_ERR_ZERO):
         raise ArithmeticError(msg)
     raise RuntimeError(msg)
+
+
+def draw_dict(r):
+    """dcs_draw_result as the primary-CDF fields of draw_checked."""
+    return {'A_index': int(r.index), 'selected_scaled_weight': r.weight, 'scaled_mass_cdf': r.total,
+            'cdf_lower_numpy': r.cdf_lower, 'cdf_upper_numpy': r.cdf_upper, 'cdf_margin': r.margin}
 
 
 def ptr(a, ctype):
@@ -159,6 +167,14 @@ class Context:
         ms = C.c_double()
         check(self.lib.dcs_stream_copy(self.handle, C.c_uint64(nbytes), int(reps), C.byref(ms)))
         return ms.value
+
+    def draw_noalloc(self, block_ptr, k, u):
+        """The CDF draw of dcs_draw on a complex128 device block of 2^k entries at block_ptr, through the
+        entry point that allocates nothing (one static scratch per device; calls on one device must not
+        overlap)."""
+        r = DrawResult()
+        check(self.lib.dcs_draw_noalloc(self.handle, C.c_void_p(int(block_ptr)), int(k), float(u), C.byref(r)))
+        return draw_dict(r)
 
     def close(self):
         if self.handle:
@@ -352,8 +368,7 @@ class Plan:
     def draw(self, block, u):
         r = DrawResult()
         check(self.lib.dcs_draw(self.handle, int(block), float(u), C.byref(r)))
-        return {'A_index': int(r.index), 'selected_scaled_weight': r.weight, 'scaled_mass_cdf': r.total,
-                'cdf_lower_numpy': r.cdf_lower, 'cdf_upper_numpy': r.cdf_upper, 'cdf_margin': r.margin}
+        return draw_dict(r)
 
     def frontier_terminal(self, lane=0):
         if self.storage == STORAGE_F16X2:

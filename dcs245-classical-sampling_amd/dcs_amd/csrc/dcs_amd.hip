@@ -506,7 +506,24 @@ __global__ void k_weights(const double2 *__restrict__ v, u64 n, double *__restri
 //   ck[j] = w[0] + ... + w[64 j + 63]  (j < n / 64),  ck[n / 64] = the total;
 // k_search recomputes the 64 sums of one segment from ck[j - 1] in the same order, so every
 // cumulative value it uses is bit-identical to cumsum's.
-__global__ void __launch_bounds__(64) k_chain(const double *__restrict__ w, u64 n, double *__restrict__ ck) {
+//
+// W is the weight source: WeightArray reads the weights k_weights wrote, BlockWeights forms them from
+// the block itself (the same unfused re*re + im*im, so the same bits) and needs no weight buffer.
+struct WeightArray {
+    const double *__restrict__ w;
+    __device__ __forceinline__ double operator()(u64 i) const { return w[i]; }
+};
+
+struct BlockWeights {
+    const double2 *__restrict__ v;
+    __device__ __forceinline__ double operator()(u64 i) const {
+        const double2 a = v[i];
+        return a.x * a.x + a.y * a.y;
+    }
+};
+
+template <class W>
+__global__ void __launch_bounds__(64) k_chain(W w, u64 n, double *__restrict__ ck) {
     __shared__ double seg[512];
     const int lane = threadIdx.x;
     const u64 nchunk = (n + 511) >> 9;
@@ -514,7 +531,7 @@ __global__ void __launch_bounds__(64) k_chain(const double *__restrict__ w, u64 
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const u64 i = (u64)k * 64 + lane;
-        r[k] = i < n ? w[i] : 0.0;
+        r[k] = i < n ? w(i) : 0.0;
     }
     double acc = 0.0;
     for (u64 c = 0; c < nchunk; ++c) {
@@ -526,7 +543,7 @@ __global__ void __launch_bounds__(64) k_chain(const double *__restrict__ w, u64 
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const u64 i = base + 512 + (u64)k * 64 + lane;
-                r[k] = i < n ? w[i] : 0.0;
+                r[k] = i < n ? w(i) : 0.0;
             }
         }
         if (base + 512 <= n) {
@@ -564,8 +581,8 @@ __global__ void __launch_bounds__(64) k_chain(const double *__restrict__ w, u64 
 // searchsorted(cdf, u, side='right') on cdf = cumsum(w) / total with cdf[-1] forced to 1
 // (sampling/sampling_protocol.py:56-65): binary search over the segment ends, then the segment's
 // running sums in order.
-__global__ void k_search(const double *__restrict__ w, const double *__restrict__ ck, u64 n, double u,
-                         DrawOut *out) {
+template <class W>
+__global__ void k_search(W w, const double *__restrict__ ck, u64 n, double u, DrawOut *out) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     DrawOut r;
     memset(&r, 0, sizeof(r));
@@ -589,7 +606,7 @@ __global__ void k_search(const double *__restrict__ w, const double *__restrict_
     double lower = lo ? acc / total : 0.0;
     u64 idx = n;
     for (u64 i = s0; i < s1; ++i) {
-        acc = acc + w[i];
+        acc = acc + w(i);
         const double c = (i == n - 1) ? 1.0 : acc / total;
         if (c > u) {
             idx = i;
@@ -604,12 +621,20 @@ __global__ void k_search(const double *__restrict__ w, const double *__restrict_
         *out = r;
         return;
     }
-    r.weight = w[idx];
+    r.weight = w(idx);
     r.lower = idx ? lower : 0.0;
     r.margin = fmin(u - r.lower, r.upper - u);
     r.status = (r.weight > 0.0) ? DCS_OK : DCS_ERR_ZERO;
     *out = r;
 }
+
+// Device scratch of dcs_draw_noalloc: k_chain's segment sums of a block of up to 2^DCS_DRAW_NOALLOC_MAX_K
+// entries (2^(max - 6) + 1 doubles, 512 KiB), then one DrawOut.  Every device the library runs on
+// holds one copy for the life of the process; host code reaches it only through the address that
+// hipGetSymbolAddress returned on that device (dcs_context::scratch).
+constexpr u64 kNoallocSums = (1ull << (DCS_DRAW_NOALLOC_MAX_K - 6)) + 1;
+static_assert(sizeof(DrawOut) <= 8 * sizeof(double), "DrawOut must fit the scratch tail");
+static __device__ __align__(256) double scratch[kNoallocSums + 8];
 
 __global__ void k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, u64 n) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x) dst[i] = src[i];
@@ -646,7 +671,42 @@ struct dcs_context {
     int device;
     hipStream_t stream;
     int cus;
+    void *scratch;      // this device's address of the static scratch (dcs_draw_noalloc)
+    DrawOut *h_draw;    // pinned result of dcs_draw_noalloc
 };
+
+namespace {
+// Device address of `scratch`, looked up once per device with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices] = {};
+
+int scratch_address(int device, void **out) {
+    if (device < 0 || device >= kMaxDevices) return fail(DCS_ERR_INVALID, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(DCS_ERR_RUNTIME, "no device address for the draw scratch on device %d", device);
+        g_scratch_addr[device] = ptr;
+    }
+    *out = g_scratch_addr[device];
+    return DCS_OK;
+}
+
+// Fill a dcs_draw_result from the device's DrawOut and map its status.
+int draw_result(const DrawOut &r, dcs_draw_result *out) {
+    out->index = r.index;
+    out->weight = r.weight;
+    out->total = r.total;
+    out->cdf_lower = r.lower;
+    out->cdf_upper = r.upper;
+    out->margin = r.margin;
+    if (r.status == DCS_ERR_NONFINITE) return fail(DCS_ERR_NONFINITE, "nonfinite block mass");
+    if (r.status != DCS_OK) return fail(DCS_ERR_ZERO, "numerically zero block or invalid CDF outcome");
+    return DCS_OK;
+}
+}  // namespace
 
 struct PendingEvent {
     int family;
@@ -1448,12 +1508,19 @@ int dcs_context_create(int device, dcs_context **out) {
     if (device < 0 || device >= n) return fail(DCS_ERR_INVALID, "device %d out of range (%d devices)", device, n);
     HIPCHK(hipSetDevice(device));
     dcs_context *c = new dcs_context();
+    std::unique_ptr<dcs_context> guard(c);
     c->device = device;
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     c->cus = prop.multiProcessorCount;
-    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    *out = c;
+    int rc = scratch_address(device, &c->scratch);
+    if (rc) return rc;
+    HIPCHK(hipHostMalloc((void **)&c->h_draw, sizeof(DrawOut), hipHostMallocDefault));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipHostFree(c->h_draw);
+        return fail(DCS_ERR_RUNTIME, "hipStreamCreateWithFlags failed on device %d", device);
+    }
+    *out = guard.release();
     return DCS_OK;
 }
 
@@ -1462,6 +1529,7 @@ void dcs_context_destroy(dcs_context *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
+    (void)hipHostFree(ctx->h_draw);
     delete ctx;
 }
 
@@ -2055,22 +2123,34 @@ int dcs_draw(dcs_plan *p, int block, double u, dcs_draw_result *out) {
     const double2 *v = p->d_blocks + ((u64)block << p->k);
     double *ck = (double *)p->d_scratch;  // n / 64 + 1 doubles
     PROF(4, (k_weights<<<grid_for(n, p->ctx->cus, 4), kThreads, 0, p->ctx->stream>>>(v, n, p->d_cum)));
-    PROF(4, (k_chain<<<1, 64, 0, p->ctx->stream>>>(p->d_cum, n, ck)));
-    PROF(4, (k_search<<<1, 64, 0, p->ctx->stream>>>(p->d_cum, ck, n, u, p->d_draw)));
+    PROF(4, (k_chain<<<1, 64, 0, p->ctx->stream>>>(WeightArray{p->d_cum}, n, ck)));
+    PROF(4, (k_search<<<1, 64, 0, p->ctx->stream>>>(WeightArray{p->d_cum}, ck, n, u, p->d_draw)));
     HIPCHK(hipMemcpyAsync(p->h_draw, p->d_draw, sizeof(DrawOut), hipMemcpyDeviceToHost, p->ctx->stream));
     HIPCHK(hipStreamSynchronize(p->ctx->stream));
     int rc = ev_resolve(p);
     if (rc) return rc;
-    const DrawOut &r = *p->h_draw;
-    out->index = r.index;
-    out->weight = r.weight;
-    out->total = r.total;
-    out->cdf_lower = r.lower;
-    out->cdf_upper = r.upper;
-    out->margin = r.margin;
-    if (r.status == DCS_ERR_NONFINITE) return fail(DCS_ERR_NONFINITE, "nonfinite block mass");
-    if (r.status != DCS_OK) return fail(DCS_ERR_ZERO, "numerically zero block or invalid CDF outcome");
-    return DCS_OK;
+    return draw_result(*p->h_draw, out);
+}
+
+int dcs_draw_noalloc(dcs_context *ctx, const void *block, int k, double u, dcs_draw_result *out) {
+    if (!ctx || !block || !out) return fail(DCS_ERR_INVALID, "bad argument");
+    if (k < 0 || k > DCS_DRAW_NOALLOC_MAX_K)
+        return fail(DCS_ERR_INVALID, "block of 2^%d entries: dcs_draw_noalloc's scratch holds blocks of at most 2^%d",
+                    k, DCS_DRAW_NOALLOC_MAX_K);
+    if (!(u >= 0.0 && u < 1.0)) return fail(DCS_ERR_INVALID, "uniform must lie in [0, 1)");
+    HIPCHK(hipSetDevice(ctx->device));
+    const u64 n = 1ull << k;
+    // every segment sum k_search reads (ck[0 .. n/64]) and the whole DrawOut are written by this
+    // call's own kernels, so nothing left by an earlier call is read
+    double *ck = (double *)ctx->scratch;
+    DrawOut *d = (DrawOut *)(ck + kNoallocSums);
+    const BlockWeights w{(const double2 *)block};
+    k_chain<<<1, 64, 0, ctx->stream>>>(w, n, ck);
+    k_search<<<1, 64, 0, ctx->stream>>>(w, ck, n, u, d);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->h_draw, d, sizeof(DrawOut), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return draw_result(*ctx->h_draw, out);
 }
 
 int dcs_frontier_read(dcs_plan *p, int lane, void *host_out) {

@@ -210,6 +210,20 @@ int dcs_block_transform(dcs_plan *plan, int block);
 int dcs_block_read(dcs_plan *plan, int block, double *host_out);
 int dcs_block_device_pointer(dcs_plan *plan, int block, void **ptr);
 int dcs_draw(dcs_plan *plan, int block, double u, dcs_draw_result *out);
+/* Largest block (log2 entries) that dcs_draw_noalloc accepts. */
+#define DCS_DRAW_NOALLOC_MAX_K 22
+
+/* dcs_draw on a block the caller holds on the context's device, allocating nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (serving loops and harnesses
+ * that forbid hipMalloc).  `block` is a device pointer to 2^k complex128 entries (interleaved re/im,
+ * e.g. from dcs_block_device_pointer) whose writes have completed; k <= DCS_DRAW_NOALLOC_MAX_K, larger
+ * blocks are rejected with DCS_ERR_INVALID.  The result and status codes equal dcs_draw's on the same
+ * block.  The segment sums and the result live in one static device array per device (512 KiB), held
+ * until the process ends and shared by every call on that device: calls that overlap on one device
+ * race, so callers serialise them.  Runs on the context's stream and returns after the device
+ * finished. */
+int dcs_draw_noalloc(dcs_context *ctx, const void *block, int k, double u, dcs_draw_result *out);
+
 /* Terminal environment of a lane (2^r entries in storage precision, interleaved re/im). */
 int dcs_frontier_read(dcs_plan *plan, int lane, void *host_out);
 
