@@ -12,7 +12,7 @@ import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / 'libdcs_amd.so'
 
-DCS_OK, DCS_ERR_INVALID, DCS_ERR_RUNTIME, DCS_ERR_NOMEM, DCS_ERR_NONFINITE, DCS_ERR_ZERO = range(6)
+DCS_OK, DCS_ERR_INVALID, DCS_ERR_RUNTIME, DCS_ERR_NOMEM, DCS_ERR_NONFINITE, DCS_ERR_ZERO, DCS_ERR_UNSUPPORTED = range(7)
 DRAW_NOALLOC_MAX_K = 22   # DCS_DRAW_NOALLOC_MAX_K
 STORAGE_C64, STORAGE_C128, STORAGE_F16X2 = 0, 1, 2
 MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED, MODE_COMPILED = 0, 1, 2, 3
@@ -58,7 +58,13 @@ class GatherDesc(C.Structure):
 
 class DrawResult(C.Structure):
     _fields_ = [('index', C.c_int64), ('weight', C.c_double), ('total', C.c_double),
-                ('cdf_lower', C.c_double), ('cdf_upper', C.c_double), ('margin', C.c_double)]
+                ('cdf_lower', C.c_double), ('cdf_upper', C.c_double), ('margin', C.c_double),
+                ('mass_check', C.c_double), ('cdf_lower_check', C.c_double), ('cdf_upper_check', C.c_double)]
+
+
+class ScheduleUnsupported(ValueError):
+    """The requested schedule cannot be built on this device (DCS_ERR_UNSUPPORTED): the generated group
+    kernels do not compile, or one of them cannot be resident.  A refusal, not a device failure."""
 
 
 _lib = None
@@ -120,15 +126,20 @@ def check(rc):
         raise ValueError(msg)
     if rc == DCS_ERR_NOMEM:
         raise MemoryError(msg)
+    if rc == DCS_ERR_UNSUPPORTED:
+        raise ScheduleUnsupported(msg)
     if rc in (DCS_ERR_NONFINITE, DCS_ERR_ZERO):
         raise ArithmeticError(msg)
     raise RuntimeError(msg)
 
 
 def draw_dict(r):
-    """dcs_draw_result as the primary-CDF fields of draw_checked."""
+    """dcs_draw_result as the fields of draw_checked it covers: the primary CDF (numpy's cumsum order) and
+    the independent interval, here summed with compensation on the device instead of math.fsum."""
     return {'A_index': int(r.index), 'selected_scaled_weight': r.weight, 'scaled_mass_cdf': r.total,
-            'cdf_lower_numpy': r.cdf_lower, 'cdf_upper_numpy': r.cdf_upper, 'cdf_margin': r.margin}
+            'cdf_lower_numpy': r.cdf_lower, 'cdf_upper_numpy': r.cdf_upper, 'cdf_margin': r.margin,
+            'scaled_mass_compensated': r.mass_check, 'cdf_lower_compensated': r.cdf_lower_check,
+            'cdf_upper_compensated': r.cdf_upper_check, 'independent_cdf_check_passed': True}
 
 
 def ptr(a, ctype):

@@ -34,9 +34,10 @@ DEFAULT_CANDIDATES = (
 _memory_cache = {}
 
 
-def plan_fingerprint(engine_cls, args, dtype, arch):
-    """SHA-256 of what fixes the schedule's behaviour: the in-place step table, the storage and the
-    device architecture."""
+def plan_fingerprint(engine_cls, args, dtype, arch, alloc_kw=None):
+    """SHA-256 of what fixes the schedule's behaviour: the in-place step table, the storage, the device
+    architecture and the allocation options every candidate shares (lane count, LDS storage, register
+    pass span, resident bits)."""
     eng = engine_cls(*args)
     t = eng.bp.step_table()
     h = hashlib.sha256()
@@ -44,6 +45,7 @@ def plan_fingerprint(engine_cls, args, dtype, arch):
         h.update(k.encode())
         h.update(np.ascontiguousarray(t[k]).tobytes())
     h.update(f'{engine_cls.__name__}|{np.dtype(dtype).name}|{arch}|{eng.bp.inplace_maxslots}'.encode())
+    h.update(json.dumps(sorted((alloc_kw or {}).items()), default=str).encode())
     return h.hexdigest()
 
 
@@ -70,7 +72,7 @@ def autotune(engine_cls, args, dtype=np.complex64, x=None, candidates=DEFAULT_CA
     if arch is None:
         from . import _native
         arch = _native.device_info(device)['arch']
-    fp = plan_fingerprint(engine_cls, args, dtype, arch)
+    fp = plan_fingerprint(engine_cls, args, dtype, arch, alloc_kw)
     disk = {}
     if cache_path is not None and Path(cache_path).exists():
         disk = json.loads(Path(cache_path).read_text())
@@ -86,8 +88,11 @@ def autotune(engine_cls, args, dtype=np.complex64, x=None, candidates=DEFAULT_CA
         try:
             eng.alloc(dtype=dtype, device=device, schedule='compiled', **{**(alloc_kw or {}), **cand})
             seconds = _time_engine(eng, x, repeats)
-        except (MemoryError, ValueError, RuntimeError) as exc:   # a candidate the plan or device refuses
-            table.append({'options': dict(cand), 'seconds': None, 'error': str(exc)})
+        except (MemoryError, ValueError) as exc:
+            # a candidate the plan or device refuses: no memory, an invalid tiling, or a schedule that does
+            # not compile or cannot be resident (ScheduleUnsupported); a RuntimeError is a device failure
+            # and ends the tuning
+            table.append({'options': dict(cand), 'seconds': None, 'error': f'{type(exc).__name__}: {exc}'})
             continue
         finally:
             eng.close()

@@ -129,9 +129,15 @@ struct GroupDev {
 struct DrawOut {
     int64_t index;
     double weight, total, lower, upper, margin;
+    double mass_check, lower_check, upper_check;  // the independent compensated-sum interval
     int32_t status;
     int32_t pad;
 };
+
+// DrawOut status beyond dcs_status: the independent interval does not contain the uniform.
+constexpr int32_t kDrawIntervalDisagrees = 100;
+// Workgroups of the draw's independent interval check (k_interval_part).
+constexpr int kIntervalBlocks = 256;
 
 // ------------------------------------------------------------------------------------------------
 // k_step: one step, one thread per orbit, orbit index deposited into the enumerated slots.
@@ -629,12 +635,116 @@ __global__ void k_search(W w, const double *__restrict__ ck, u64 n, double u, Dr
 }
 
 // Device scratch of dcs_draw_noalloc: k_chain's segment sums of a block of up to 2^DCS_DRAW_NOALLOC_MAX_K
-// entries (2^(max - 6) + 1 doubles, 512 KiB), then one DrawOut.  Every device the library runs on
+// entries (2^(max - 6) + 1 doubles, 512 KiB), then one DrawOut, then the interval check's partial sums
+// (8 KiB).  Every device the library runs on
 // holds one copy for the life of the process; host code reaches it only through the address that
 // hipGetSymbolAddress returned on that device (dcs_context::scratch).
 constexpr u64 kNoallocSums = (1ull << (DCS_DRAW_NOALLOC_MAX_K - 6)) + 1;
-static_assert(sizeof(DrawOut) <= 8 * sizeof(double), "DrawOut must fit the scratch tail");
-static __device__ __align__(256) double scratch[kNoallocSums + 8];
+constexpr u64 kNoallocDrawOut = kNoallocSums + 3;    // 32-byte aligned offset of the DrawOut (in doubles)
+constexpr u64 kNoallocParts = kNoallocDrawOut + 16;  // the interval check's partial sums
+static_assert(sizeof(DrawOut) <= 16 * sizeof(double), "DrawOut must fit its scratch slot");
+static __device__ __align__(256) double scratch[kNoallocParts + 4 * kIntervalBlocks];
+
+// Independent check of the drawn interval (draw_checked's math.fsum interval,
+// sampling/sampling_protocol.py:66-72): the prefix mass below the drawn index and the total mass are
+// summed again, in a different order and with compensation, and [prefix, prefix + w) / total must
+// contain the uniform.  Each sum is Ogita, Rump and Oishi's Sum2 ("Accurate sum and dot product",
+// SIAM J. Sci. Comput. 26, 2005): Knuth's error-free TwoSum keeps every rounding error, the errors
+// are added up separately and folded in at the end, so the result is as accurate as a sum in twice
+// the working precision (within one unit in the last place of the exactly rounded sum that math.fsum
+// returns).  Partial sums of kIntervalBlocks workgroups are merged in a fixed order by one wave.
+
+struct Sum2 {
+    double s, c;  // value s + c
+};
+
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+    s = a + b;
+    const double bv = s - a;
+    e = (a - (s - bv)) + (b - bv);
+}
+
+__device__ __forceinline__ void sum2_add(Sum2 &a, double x) {
+    double s, e;
+    two_sum(a.s, x, s, e);
+    a.s = s;
+    a.c += e;
+}
+
+__device__ __forceinline__ Sum2 sum2_merge(const Sum2 &a, const Sum2 &b) {
+    double s, e;
+    two_sum(a.s, b.s, s, e);
+    return Sum2{s, (a.c + b.c) + e};
+}
+
+__device__ __forceinline__ Sum2 sum2_wave(Sum2 v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const Sum2 o{__shfl_xor(v.s, off, 64), __shfl_xor(v.c, off, 64)};
+        v = sum2_merge(v, o);
+    }
+    return v;
+}
+
+// part[4 b .. 4 b + 3] = prefix (s, c), total (s, c) of workgroup b; kIntervalBlocks x 256 threads.
+template <class W>
+__global__ void __launch_bounds__(256) k_interval_part(W w, u64 n, const DrawOut *__restrict__ d,
+                                                       double *__restrict__ part) {
+    __shared__ double red[4][4];
+    const u64 end = d->status == DCS_OK ? (u64)d->index : 0;
+    Sum2 pre{0.0, 0.0}, tot{0.0, 0.0};
+    for (u64 i = (u64)blockIdx.x * 256 + threadIdx.x; i < n; i += (u64)256 * gridDim.x) {
+        const double x = w(i);
+        sum2_add(tot, x);
+        if (i < end) sum2_add(pre, x);
+    }
+    pre = sum2_wave(pre);
+    tot = sum2_wave(tot);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave][0] = pre.s;
+        red[wave][1] = pre.c;
+        red[wave][2] = tot.s;
+        red[wave][3] = tot.c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Sum2 p{red[0][0], red[0][1]}, t{red[0][2], red[0][3]};
+        for (int k = 1; k < 4; ++k) {
+            p = sum2_merge(p, Sum2{red[k][0], red[k][1]});
+            t = sum2_merge(t, Sum2{red[k][2], red[k][3]});
+        }
+        double *o = part + 4 * blockIdx.x;
+        o[0] = p.s;
+        o[1] = p.c;
+        o[2] = t.s;
+        o[3] = t.c;
+    }
+}
+
+// Offset (in doubles) of the interval check's partial sums in a plan's draw scratch, after the n / 64 + 1
+// segment sums of k_chain; and the scratch bytes a block of n entries needs.
+inline u64 draw_parts_offset(u64 n) { return ((n >> 6) + 1 + 3) & ~3ull; }
+inline u64 draw_scratch_bytes(u64 n) { return std::max<u64>(16 * n, 8 * (draw_parts_offset(n) + 4 * kIntervalBlocks)); }
+
+// One wave: merge the partial sums and check the interval of the drawn index.
+template <class W>
+__global__ void __launch_bounds__(64) k_interval_check(W w, const double *__restrict__ part, double u,
+                                                       DrawOut *__restrict__ d) {
+    Sum2 pre{0.0, 0.0}, tot{0.0, 0.0};
+    for (int b = threadIdx.x; b < kIntervalBlocks; b += 64) {
+        pre = sum2_merge(pre, Sum2{part[4 * b], part[4 * b + 1]});
+        tot = sum2_merge(tot, Sum2{part[4 * b + 2], part[4 * b + 3]});
+    }
+    pre = sum2_wave(pre);
+    tot = sum2_wave(tot);
+    if (threadIdx.x != 0 || d->status != DCS_OK) return;
+    const double prefix = pre.s + pre.c, total = tot.s + tot.c;
+    const double lower = prefix / total, upper = (prefix + w((u64)d->index)) / total;
+    d->mass_check = total;
+    d->lower_check = lower;
+    d->upper_check = upper;
+    if (!(lower <= u && u < upper)) d->status = kDrawIntervalDisagrees;
+}
 
 __global__ void k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, u64 n) {
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x) dst[i] = src[i];
@@ -702,6 +812,12 @@ int draw_result(const DrawOut &r, dcs_draw_result *out) {
     out->cdf_lower = r.lower;
     out->cdf_upper = r.upper;
     out->margin = r.margin;
+    out->mass_check = r.mass_check;
+    out->cdf_lower_check = r.lower_check;
+    out->cdf_upper_check = r.upper_check;
+    if (r.status == kDrawIntervalDisagrees)
+        return fail(DCS_ERR_ZERO, "independent compensated CDF interval [%.17g, %.17g) of index %lld does not contain "
+                    "the uniform; the draw is refused, not repeated", r.lower_check, r.upper_check, (long long)r.index);
     if (r.status == DCS_ERR_NONFINITE) return fail(DCS_ERR_NONFINITE, "nonfinite block mass");
     if (r.status != DCS_OK) return fail(DCS_ERR_ZERO, "numerically zero block or invalid CDF outcome");
     return DCS_OK;
@@ -1356,7 +1472,7 @@ int compile_groups(dcs_plan *p, double *seconds) {
                 std::string log(ls, '\0');
                 if (ls) hiprtcGetProgramLog(prog, &log[0]);
                 hiprtcDestroyProgram(&prog);
-                return fail(DCS_ERR_RUNTIME, "hipRTC compile of the group kernels failed: %.900s", log.c_str());
+                return fail(DCS_ERR_UNSUPPORTED, "hipRTC compile of the group kernels failed: %.900s", log.c_str());
             }
             size_t cs = 0;
             hiprtcGetCodeSize(prog, &cs);
@@ -1377,7 +1493,7 @@ int compile_groups(dcs_plan *p, double *seconds) {
         p->group_threads[gi] = codegen::group_threads(p->h_groups[gi]);
         int nb = 0;
         HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, p->group_fn[gi], p->group_threads[gi], 0));
-        if (nb < 1) return fail(DCS_ERR_RUNTIME, "group kernel %zu cannot be resident (registers or LDS)", gi);
+        if (nb < 1) return fail(DCS_ERR_UNSUPPORTED, "group kernel %zu cannot be resident (registers or LDS)", gi);
         const u64 cap = (u64)p->ctx->cus * (u64)std::max(1, nb);
         p->group_grid[gi] = (unsigned)std::max<u64>(1, std::min<u64>(p->h_groups[gi].ntiles, cap));
     }
@@ -1723,7 +1839,7 @@ int dcs_plan_create(dcs_context *ctx, const dcs_plan_desc *d, dcs_plan **out) {
     DALLOC(p->d_phase, sizeof(double) * block_len);
     DALLOC(p->d_blocks, sizeof(double2) * block_len * d->blocks);
     DALLOC(p->d_cum, sizeof(double) * block_len);
-    DALLOC(p->d_scratch, sizeof(double2) * block_len);
+    DALLOC(p->d_scratch, draw_scratch_bytes(block_len));
     DALLOC(p->d_draw, sizeof(DrawOut));
     HALLOC(p->h_dyn, sizeof(Dyn) * (size_t)d->lanes * d->nsteps);
     HALLOC(p->h_lanes, sizeof(int) * (size_t)d->lanes);
@@ -1835,7 +1951,7 @@ int dcs_gather_plan_create(dcs_context *ctx, const dcs_gather_desc *d, dcs_plan 
     DALLOC(p->d_phase, sizeof(double) * block_len);
     DALLOC(p->d_blocks, sizeof(double2) * block_len * d->blocks);
     DALLOC(p->d_cum, sizeof(double) * block_len);
-    DALLOC(p->d_scratch, sizeof(double2) * block_len);
+    DALLOC(p->d_scratch, draw_scratch_bytes(block_len));
     DALLOC(p->d_draw, sizeof(DrawOut));
     HALLOC(p->h_dyn, sizeof(Dyn) * (size_t)d->nsteps);
     HALLOC(p->h_lanes, sizeof(int));
@@ -2125,6 +2241,9 @@ int dcs_draw(dcs_plan *p, int block, double u, dcs_draw_result *out) {
     PROF(4, (k_weights<<<grid_for(n, p->ctx->cus, 4), kThreads, 0, p->ctx->stream>>>(v, n, p->d_cum)));
     PROF(4, (k_chain<<<1, 64, 0, p->ctx->stream>>>(WeightArray{p->d_cum}, n, ck)));
     PROF(4, (k_search<<<1, 64, 0, p->ctx->stream>>>(WeightArray{p->d_cum}, ck, n, u, p->d_draw)));
+    double *part = ck + draw_parts_offset(n);
+    PROF(4, (k_interval_part<<<kIntervalBlocks, 256, 0, p->ctx->stream>>>(WeightArray{p->d_cum}, n, p->d_draw, part)));
+    PROF(4, (k_interval_check<<<1, 64, 0, p->ctx->stream>>>(WeightArray{p->d_cum}, part, u, p->d_draw)));
     HIPCHK(hipMemcpyAsync(p->h_draw, p->d_draw, sizeof(DrawOut), hipMemcpyDeviceToHost, p->ctx->stream));
     HIPCHK(hipStreamSynchronize(p->ctx->stream));
     int rc = ev_resolve(p);
@@ -2140,13 +2259,16 @@ int dcs_draw_noalloc(dcs_context *ctx, const void *block, int k, double u, dcs_d
     if (!(u >= 0.0 && u < 1.0)) return fail(DCS_ERR_INVALID, "uniform must lie in [0, 1)");
     HIPCHK(hipSetDevice(ctx->device));
     const u64 n = 1ull << k;
-    // every segment sum k_search reads (ck[0 .. n/64]) and the whole DrawOut are written by this
-    // call's own kernels, so nothing left by an earlier call is read
+    // every segment sum k_search reads (ck[0 .. n/64]), the whole DrawOut and every partial sum of the
+    // interval check are written by this call's own kernels, so nothing left by an earlier call is read
     double *ck = (double *)ctx->scratch;
-    DrawOut *d = (DrawOut *)(ck + kNoallocSums);
+    DrawOut *d = (DrawOut *)(ck + kNoallocDrawOut);
+    double *part = ck + kNoallocParts;
     const BlockWeights w{(const double2 *)block};
     k_chain<<<1, 64, 0, ctx->stream>>>(w, n, ck);
     k_search<<<1, 64, 0, ctx->stream>>>(w, ck, n, u, d);
+    k_interval_part<<<kIntervalBlocks, 256, 0, ctx->stream>>>(w, n, d, part);
+    k_interval_check<<<1, 64, 0, ctx->stream>>>(w, part, u, d);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->h_draw, d, sizeof(DrawOut), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));

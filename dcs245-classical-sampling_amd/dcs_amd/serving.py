@@ -65,13 +65,36 @@ def footprint_bytes(bp, dtype, lanes, blocks):
 
 
 class Circuit:
+    """A resident circuit: its engine holds frontiers, plan tables and block accumulators on the device."""
+
     def __init__(self, name, engine, lanes, max_batch, nbytes):
         self.name, self.engine, self.lanes, self.max_batch, self.nbytes = name, engine, lanes, max_batch, nbytes
         self.served = 0
+        self.last_used = time.perf_counter()
+
+
+@dataclass
+class CircuitSpec:
+    """A circuit the server knows: its host plan is built, its device memory is not."""
+    name: str
+    args: tuple                     # engine arguments (graph, unitaries, n, order, parity ops, open vertices)
+    engine: object                  # the engine object (allocated while resident)
+    dtype: object
+    lanes: int
+    max_batch: int
+    alloc_kw: dict
+    autotune: bool = False
+    tune_cache: object = None
+    admissions: int = 0
 
 
 class Server:
-    """Resident circuits and the batching loop on one device."""
+    """Circuits and the batching loop on one device.
+
+    add() makes a circuit known (host plan only); it becomes resident -- frontiers, tables and blocks
+    allocated -- when a request for it is admitted, and an idle resident circuit (nothing queued, no batch
+    in flight) is evicted, least recently used first, when a waiting circuit needs its memory.
+    register() adds a circuit and admits it at once."""
 
     def __init__(self, device=0, hbm_budget=None, reserve=8 << 30, engine_factory=None):
         self.device = device
@@ -79,59 +102,123 @@ class Server:
             hbm_budget = _native.device_info(device)['free_bytes'] - reserve
         self.budget = int(hbm_budget)
         self.used = 0
+        self.catalog = collections.OrderedDict()
         self.circuits = collections.OrderedDict()
+        self.evictions = 0
         self._factory = engine_factory or InplaceConditionalBlockGPU
 
     # ---- admission ---------------------------------------------------------------------------
+    def add(self, name, graph, unitaries, n, order, parity_operations, open_vertices, dtype=np.complex64,
+            lanes=4, max_batch=4, autotune=False, tune_cache=None, **alloc_kw):
+        """Make a circuit known without allocating device memory.  Raises MemoryError when even one lane
+        exceeds the whole budget (it could never be admitted)."""
+        if name in self.catalog:
+            raise ValueError(f'circuit {name!r} already known')
+        args = (graph, unitaries, n, order, parity_operations, open_vertices)
+        engine = self._factory(*args)
+        need = footprint_bytes(engine.bp, dtype, 1, max_batch)
+        if need > self.budget:
+            raise MemoryError(f'circuit {name!r} needs {need} device bytes with one lane, more than the whole '
+                              f'{self.budget}-byte budget')
+        spec = CircuitSpec(name, args, engine, dtype, max(1, int(lanes)), int(max_batch), dict(alloc_kw), autotune,
+                           tune_cache)
+        self.catalog[name] = spec
+        return spec
+
     def register(self, name, graph, unitaries, n, order, parity_operations, open_vertices, dtype=np.complex64,
                  lanes=4, max_batch=4, autotune=False, tune_cache=None, **alloc_kw):
-        """Admit a circuit.  autotune=True first picks the compiled schedule's tiling for this plan by
-        measurement (dcs_amd.autotune; cached per plan fingerprint, optionally in tune_cache)."""
+        """Add a circuit and admit it now.  The lane count is halved until it fits what the resident
+        circuits leave of the budget; MemoryError if one lane does not fit.  autotune=True first picks the
+        compiled schedule's tiling for this plan by measurement (dcs_amd.autotune; cached per plan
+        fingerprint, optionally in tune_cache)."""
+        self.add(name, graph, unitaries, n, order, parity_operations, open_vertices, dtype=dtype, lanes=lanes,
+                 max_batch=max_batch, autotune=autotune, tune_cache=tune_cache, **alloc_kw)
+        c = self._admit(name)
+        if c is None:
+            spec = self.catalog.pop(name)
+            need = footprint_bytes(spec.engine.bp, dtype, 1, max_batch)
+            raise MemoryError(f'circuit {name!r} needs {need} device bytes with one lane; '
+                              f'{self.budget - self.used} of the {self.budget}-byte budget are left')
+        return c
+
+    def _fitting_lanes(self, spec, left):
+        lanes = spec.lanes
+        while lanes >= 1 and footprint_bytes(spec.engine.bp, spec.dtype, lanes, spec.max_batch) > left:
+            lanes //= 2
+        return lanes
+
+    def _victims(self, name, idle):
+        """Idle resident circuits to evict (least recently used first) so that `name` fits with one lane,
+        or None when evicting all of them is not enough."""
+        spec = self.catalog[name]
+        need = footprint_bytes(spec.engine.bp, spec.dtype, 1, spec.max_batch)
+        left = self.budget - self.used
+        out = []
+        for c in sorted((self.circuits[v] for v in idle), key=lambda c: c.last_used):
+            if left >= need:
+                break
+            out.append(c.name)
+            left += c.nbytes
+        return out if left >= need else None
+
+    def _admit(self, name, idle=()):
+        """Allocate a known circuit, evicting idle circuits (names in `idle`) if it needs their memory.
+        Returns the resident Circuit, or None when it does not fit."""
+        spec = self.catalog[name]
         if name in self.circuits:
-            raise ValueError(f'circuit {name!r} already registered')
-        args = (graph, unitaries, n, order, parity_operations, open_vertices)
-        if autotune:
+            return self.circuits[name]
+        victims = self._victims(name, idle)
+        if victims is None:
+            return None
+        for v in victims:
+            self._evict(v)
+        lanes = self._fitting_lanes(spec, self.budget - self.used)
+        alloc_kw = dict(spec.alloc_kw)
+        if spec.autotune:
+            # candidates are timed with the lane and block counts this circuit is admitted with
             from .autotune import autotune as tune
             shared = {k: v for k, v in alloc_kw.items() if k in ('lds_storage', 'pass_max_dim', 'resident_bits')}
-            res = tune(self._factory, args, dtype=dtype, device=self.device, cache_path=tune_cache, alloc_kw=shared)
+            shared.update(lanes=lanes, blocks=spec.max_batch)
+            res = tune(self._factory, spec.args, dtype=spec.dtype, device=self.device, cache_path=spec.tune_cache,
+                       alloc_kw=shared)
             alloc_kw = {**alloc_kw, **res['best'], 'schedule': 'compiled'}
-        engine = self._factory(*args)
-        left = self.budget - self.used
-        lanes = max(1, int(lanes))
-        while lanes >= 1 and footprint_bytes(engine.bp, dtype, lanes, max_batch) > left:
-            lanes //= 2
-        if lanes < 1:
-            need = footprint_bytes(engine.bp, dtype, 1, max_batch)
-            raise MemoryError(f'circuit {name!r} needs {need} device bytes with one lane; {left} of the '
-                              f'{self.budget}-byte budget are left')
-        engine.alloc(dtype=dtype, device=self.device, lanes=lanes, blocks=max_batch, **alloc_kw)
-        nbytes = engine.device_bytes() or footprint_bytes(engine.bp, dtype, lanes, max_batch)
+        spec.engine.alloc(dtype=spec.dtype, device=self.device, lanes=lanes, blocks=spec.max_batch, **alloc_kw)
+        nbytes = spec.engine.device_bytes() or footprint_bytes(spec.engine.bp, spec.dtype, lanes, spec.max_batch)
         self.used += nbytes
-        self.circuits[name] = Circuit(name, engine, lanes, max_batch, nbytes)
-        return self.circuits[name]
+        spec.admissions += 1
+        c = Circuit(name, spec.engine, lanes, spec.max_batch, nbytes)
+        self.circuits[name] = c
+        return c
 
-    def unregister(self, name):
+    def _evict(self, name):
         c = self.circuits.pop(name)
         c.engine.close()
         self.used -= c.nbytes
+        self.evictions += 1
+
+    def unregister(self, name):
+        if name in self.circuits:
+            self._evict(name)
+        self.catalog.pop(name, None)
 
     def close(self):
-        for name in list(self.circuits):
+        for name in list(self.catalog):
             self.unregister(name)
 
     # ---- the loop ------------------------------------------------------------------------------
     def serve(self, requests, realtime=True, keep_blocks=False, concurrent=False):
         """Serve requests (any order; each has an arrival time in seconds from now).  With realtime=False
         idle gaps are skipped instead of slept through (latencies are then measured on a clock that
-        jumps to the next arrival when nothing is waiting).  concurrent=True serves every circuit from
-        its own worker thread (arrivals in real time).  Returns the requests, filled in."""
+        jumps to the next arrival when nothing is waiting).  concurrent=True serves every resident circuit
+        from its own worker thread (arrivals in real time).  Requests for a circuit that is not resident
+        wait until it is admitted.  Returns the requests, filled in."""
         for r in requests:
-            if r.circuit not in self.circuits:
+            if r.circuit not in self.catalog:
                 raise KeyError(f'request {r.rid}: unknown circuit {r.circuit!r}')
         if concurrent:
             return self._serve_concurrent(requests, keep_blocks)
         todo = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        queues = {name: collections.deque() for name in self.circuits}
+        queues = {name: collections.deque() for name in self.catalog}
         t0 = time.perf_counter()
         skew = 0.0  # idle time skipped (realtime=False)
         clock = lambda: time.perf_counter() - t0 + skew
@@ -141,7 +228,7 @@ class Server:
             while i < len(todo) and todo[i].arrival <= now:
                 queues[todo[i].circuit].append(todo[i])
                 i += 1
-            waiting = [q for q in queues.values() if q]
+            waiting = sorted((q for q in queues.values() if q), key=lambda q: (q[0].arrival, q[0].rid))
             if not waiting:
                 gap = todo[i].arrival - now
                 if realtime:
@@ -149,15 +236,24 @@ class Server:
                 else:
                     skew += max(0.0, gap)
                 continue
-            q = min(waiting, key=lambda q: (q[0].arrival, q[0].rid))
-            c = self.circuits[q[0].circuit]
+            c = None
+            for q in waiting:       # the oldest waiting request whose circuit is, or can be made, resident
+                idle = [name for name in self.circuits if not queues[name]]
+                c = self._admit(q[0].circuit, idle)
+                if c is not None:
+                    break
+            if c is None:
+                raise RuntimeError('no waiting circuit can be admitted although every resident one is idle')
             batch = [q.popleft() for _ in range(min(c.max_batch, len(q)))]
             self._run_batch(c, batch, clock, nbatch, keep_blocks)
             nbatch += 1
         return requests
 
-
     def _run_batch(self, c, batch, clock, nbatch, keep_blocks):
+        """Contract a batch of one circuit's requests together and draw each one on the device: the draw
+        follows draw_checked (sampling/sampling_protocol.py:44-88) -- numpy's cumsum order for the index,
+        and an independent, compensated sum of the drawn interval that must contain the uniform, else
+        ArithmeticError and no result (never a second draw)."""
         start = clock()
         vecs, _ = c.engine.block_many([r.x_bits for r in batch])
         for r, v in zip(batch, vecs):
@@ -167,52 +263,94 @@ class Server:
             r.index = r.draw['A_index']
             r.start, r.done, r.batch = start, clock(), nbatch
         c.served += len(batch)
+        c.last_used = time.perf_counter()
 
     def _serve_concurrent(self, requests, keep_blocks):
         todo = sorted(requests, key=lambda r: (r.arrival, r.rid))
         cv = threading.Condition()
-        queues = {name: collections.deque() for name in self.circuits}
+        queues = {name: collections.deque() for name in self.catalog}
+        running, stop, workers = set(), set(), {}
         state = {'released': 0, 'batches': 0, 'error': None}
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0
 
-        def worker(name):
-            c = self.circuits[name]
+        def worker(name, c):
             while True:
                 with cv:
-                    cv.wait_for(lambda: queues[name] or state['released'] == len(todo) or state['error'])
-                    if state['error'] or (not queues[name] and state['released'] == len(todo)):
+                    cv.wait_for(lambda: queues[name] or name in stop or state['error']
+                                or state['released'] == len(todo))
+                    if state['error'] or name in stop or not queues[name]:
                         return
                     q = queues[name]
                     batch = [q.popleft() for _ in range(min(c.max_batch, len(q)))]
                     nb = state['batches']
                     state['batches'] += 1
+                    running.add(name)
                 try:
                     self._run_batch(c, batch, clock, nb, keep_blocks)
                 except BaseException as exc:     # stop every worker, re-raise in the caller
                     with cv:
                         state['error'] = exc
+                        running.discard(name)
                         cv.notify_all()
                     return
+                with cv:
+                    running.discard(name)
+                    cv.notify_all()
 
-        threads = [threading.Thread(target=worker, args=(name,), daemon=True) for name in self.circuits]
-        for t in threads:
+        def start(name):
+            t = threading.Thread(target=worker, args=(name, self.circuits[name]), daemon=True)
+            workers[name] = t
             t.start()
-        for r in todo:
-            wait = r.arrival - clock()
-            if wait > 0:
-                time.sleep(wait)
+
+        for name in self.circuits:
+            start(name)
+        i = 0
+        try:
+            while True:
+                with cv:
+                    if state['error']:
+                        break
+                    now = clock()
+                    while i < len(todo) and todo[i].arrival <= now:
+                        queues[todo[i].circuit].append(todo[i])
+                        state['released'] += 1
+                        i += 1
+                        cv.notify_all()
+                    pending = sorted((q for name, q in queues.items() if q and name not in self.circuits),
+                                     key=lambda q: (q[0].arrival, q[0].rid))
+                    if i == len(todo) and not pending:
+                        break
+                    idle = [n for n in self.circuits if not queues[n] and n not in running]
+                    plan = None
+                    for q in pending:
+                        victims = self._victims(q[0].circuit, idle)
+                        if victims is not None:
+                            plan = (q[0].circuit, victims)
+                            break
+                    if plan is None:
+                        # wait for the next arrival or for a worker to finish a batch
+                        gap = todo[i].arrival - now if i < len(todo) else 0.05
+                        cv.wait(timeout=max(1e-4, min(gap, 0.05)))
+                        continue
+                    stop.update(plan[1])
+                    cv.notify_all()
+                # outside the lock: let the evicted circuits' workers leave, free them, admit the new one
+                name, victims = plan
+                for v in victims:
+                    workers.pop(v).join()
+                    self._evict(v)
+                    stop.discard(v)
+                self._admit(name)
+                with cv:
+                    start(name)
+                    cv.notify_all()
+        finally:
             with cv:
-                if state['error']:
-                    break
-                queues[r.circuit].append(r)
-                state['released'] += 1
+                state['released'] = len(todo)
                 cv.notify_all()
-        with cv:
-            state['released'] = len(todo)
-            cv.notify_all()
-        for t in threads:
-            t.join()
+            for t in list(workers.values()):
+                t.join()
         if state['error'] is not None:
             raise state['error']
         return requests

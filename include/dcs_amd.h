@@ -44,7 +44,11 @@ typedef enum {
     DCS_ERR_RUNTIME = 2,    /* HIP runtime failure (RuntimeError) */
     DCS_ERR_NOMEM = 3,      /* frontier does not fit the free device memory (MemoryError) */
     DCS_ERR_NONFINITE = 4,  /* nonfinite frontier or block (ArithmeticError) */
-    DCS_ERR_ZERO = 5        /* zero / invalid CDF outcome (ArithmeticError) */
+    DCS_ERR_ZERO = 5,       /* zero / invalid CDF outcome, or the independent CDF check disagrees
+                               (ArithmeticError) */
+    DCS_ERR_UNSUPPORTED = 6 /* the schedule cannot be built on this device: the generated group
+                               kernels do not compile, or one cannot be resident (registers, LDS)
+                               (ScheduleUnsupported, a ValueError) */
 } dcs_status;
 
 typedef struct dcs_context dcs_context;
@@ -124,6 +128,12 @@ typedef struct {
     double cdf_lower;       /* normalised cdf[index-1] (0 for index 0) */
     double cdf_upper;       /* normalised cdf[index] (cdf[-1] forced to 1) */
     double margin;          /* min(u - lower, upper - u) */
+    /* The independent check of draw_checked (math.fsum interval): the total and the interval of the
+     * drawn index summed again in another order with compensated (Sum2) arithmetic.  A draw whose
+     * check interval does not contain u fails with DCS_ERR_ZERO. */
+    double mass_check;
+    double cdf_lower_check;
+    double cdf_upper_check;
 } dcs_draw_result;
 
 const char *dcs_last_error(void);

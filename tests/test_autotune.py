@@ -72,3 +72,35 @@ def test_fingerprint_depends_on_plan_storage_and_arch(args):
     assert base != at.plan_fingerprint(FakeEngine, args, np.complex64, 'gfx942')
     other = args[:5] + ([7, 9, 10, 11],)
     assert base != at.plan_fingerprint(FakeEngine, other, np.complex64, 'gfx950')
+
+
+class FaultingEngine(FakeEngine):
+    def alloc(self, dtype, device, schedule, **kw):
+        if kw.get('group_policy') == 2:
+            raise RuntimeError('hipModuleLaunchKernel failed: an illegal memory access was encountered')
+        if kw.get('fuse_max_dim') == 9:
+            from dcs_amd._native import ScheduleUnsupported
+            raise ScheduleUnsupported('group kernel 3 cannot be resident (registers or LDS)')
+        super().alloc(dtype, device, schedule, **kw)
+
+
+def test_device_failure_ends_tuning_and_refusals_are_recorded(args):
+    """A RuntimeError (a device failure) propagates instead of being logged as a refused candidate; a
+    schedule the device cannot build (ScheduleUnsupported) is recorded and skipped."""
+    at._memory_cache.clear()
+    with pytest.raises(RuntimeError, match='illegal memory access'):
+        at.autotune(FaultingEngine, args, arch='gfx950')
+    at._memory_cache.clear()
+    cands = [c for c in at.DEFAULT_CANDIDATES if c['group_policy'] != 2]
+    res = at.autotune(FaultingEngine, args, arch='gfx950', candidates=cands)
+    refused = [r for r in res['table'] if r['seconds'] is None]
+    assert refused and all(r['error'].startswith(('ScheduleUnsupported', 'MemoryError')) for r in refused)
+
+
+def test_fingerprint_depends_on_shared_allocation_options(args):
+    one = at.plan_fingerprint(FakeEngine, args, np.complex64, 'gfx950', {'lanes': 1, 'blocks': 1})
+    two = at.plan_fingerprint(FakeEngine, args, np.complex64, 'gfx950', {'lanes': 2, 'blocks': 1})
+    assert one != two
+    assert one == at.plan_fingerprint(FakeEngine, args, np.complex64, 'gfx950', {'blocks': 1, 'lanes': 1})
+    assert one != at.plan_fingerprint(FakeEngine, args, np.complex64, 'gfx950', {'lanes': 1, 'blocks': 1,
+                                                                                 'lds_storage': False})

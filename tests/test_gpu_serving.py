@@ -1,5 +1,7 @@
 """Serving loop on the device: a Poisson mix of three resident circuits; every request's block and draw
 equal serving it alone (bit for bit), and the draws agree with the host draw_checked."""
+import math
+
 import numpy as np
 import pytest
 
@@ -8,6 +10,20 @@ from dcs_amd.serving import Server, latency_stats, poisson_workload
 from dcs_amd.stabilizer import synthetic_instance
 
 pytestmark = pytest.mark.gpu
+
+
+def assert_draw_matches_host(d, block, u):
+    """A served draw against draw_checked on the host copy of its block: the same index and primary
+    (numpy) interval bit for bit, and the device's compensated interval within one unit in the last place
+    of the math.fsum interval."""
+    h = draw_checked(block, u)
+    assert d['A_index'] == h['A_index']
+    for k in ('cdf_lower_numpy', 'cdf_upper_numpy', 'scaled_mass_cdf', 'selected_scaled_weight'):
+        assert d[k] == h[k], k
+    for dk, hk in (('scaled_mass_compensated', 'scaled_mass_fsum'), ('cdf_lower_compensated', 'cdf_lower_fsum'),
+                   ('cdf_upper_compensated', 'cdf_upper_fsum')):
+        assert abs(d[dk] - h[hk]) <= math.ulp(h[hk]), (dk, d[dk], h[hk])
+    assert d['independent_cdf_check_passed'] and h['independent_cdf_check_passed']
 
 
 @pytest.mark.parametrize('concurrent', [False, True])
@@ -40,9 +56,8 @@ def test_mixed_serving_equals_alone(concurrent):
         alone = np.asarray(vec)
         assert np.array_equal(alone, r.block), r.rid
         d = vec.draw(r.uniform)
-        assert d['A_index'] == r.index
-        h = draw_checked(alone, r.uniform)
-        assert h['A_index'] == r.index and h['cdf_lower_numpy'] == d['cdf_lower_numpy']
+        assert d == r.draw
+        assert_draw_matches_host(r.draw, alone, r.uniform)
     srv.close()
 
 

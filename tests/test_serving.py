@@ -138,6 +138,51 @@ def test_concurrent_serving_propagates_engine_errors():
         srv.serve([Request(i, 'a', x, .5, 0.0) for i in range(3)], concurrent=True)
 
 
+class TrackedEngine(SlowEngine):
+    """Stand-in that records the device bytes held at once (through the server's accounting)."""
+    server, peak_used, live = None, 0, set()
+
+    def alloc(self, dtype, device, lanes, blocks, **kw):
+        super().alloc(dtype, device, lanes, blocks, **kw)
+        TrackedEngine.live.add(id(self))
+
+    def block_many(self, xs):
+        TrackedEngine.peak_used = max(TrackedEngine.peak_used, TrackedEngine.server.used)
+        assert id(self) in TrackedEngine.live        # never a batch on an evicted engine
+        return super().block_many(xs)
+
+    def close(self):
+        TrackedEngine.live.discard(id(self))
+
+
+@pytest.mark.parametrize('concurrent', [False, True])
+def test_requests_admit_circuits_as_memory_frees(concurrent):
+    """Three known circuits, of which only two fit the budget at once: requests for a circuit that is not
+    resident wait, an idle circuit (nothing queued, no batch in flight) is evicted least recently used
+    first, every request is served with the right result, and the budget is never exceeded."""
+    SlowEngine.log, SlowEngine.active, SlowEngine.peak = [], 0, 0
+    per_lane = footprint_bytes(_BP(2, 30), np.complex64, 1, 2)
+    srv = Server(hbm_budget=2 * per_lane + (1 << 20), engine_factory=TrackedEngine)
+    TrackedEngine.server, TrackedEngine.peak_used, TrackedEngine.live = srv, 0, set()
+    for name in ('a', 'b', 'c'):
+        srv.add(name, 30, None, 8, None, [], [0, 1], lanes=4, max_batch=2)
+    assert srv.used == 0 and not srv.circuits
+    with pytest.raises(MemoryError, match='whole'):
+        srv.add('huge', 32, None, 8, None, [], [0, 1], lanes=1, max_batch=2)
+    rng = np.random.default_rng(5)
+    names = ['a', 'b', 'c', 'a', 'c', 'b', 'b', 'a', 'c', 'c', 'a', 'b']
+    reqs = [Request(i, nm, rng.integers(0, 2, 8, dtype=np.uint8), float(rng.random()), 0.004 * i)
+            for i, nm in enumerate(names)]
+    want = {r.rid: r.x_bits.sum() * 1000 + int(r.uniform * 1000) for r in reqs}
+    srv.serve(reqs, realtime=concurrent, concurrent=concurrent)
+    assert all(r.index == want[r.rid] and r.done >= r.arrival for r in reqs)
+    assert srv.evictions >= 1 and len(srv.circuits) <= 2
+    assert TrackedEngine.peak_used <= srv.budget and srv.used <= srv.budget
+    assert all(srv.catalog[nm].admissions >= 1 for nm in 'abc')
+    srv.close()
+    assert srv.used == 0
+
+
 def test_poisson_workload_is_seeded_and_weighted():
     mix = {'a': (3, 10, [8, 9]), 'b': (1, 12, [0])}
     r1 = poisson_workload(mix, rate=50.0, count=400, seed=7)
