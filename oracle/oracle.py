@@ -9,10 +9,11 @@ What each function restates and what pins it:
                      with the GPU driver's scale chain (kernel_opt/inplace_gpu.py:107-123); storage 2 is
                      the binary16-pair storage of kernel_opt/inplace_half_gpu.py:14-78, whose
                      conversions (f32_to_h16, h16_to_f32) are pinned against numpy's float16 over every
-                     half and over float32 boundary and random patterns.  The binary16 contraction is
-                     pinned only to binary16 accuracy (C1 against the dense state vector): the reference
-                     holds no half-engine results, so where its roundings fall beyond that accuracy is
-                     parity unpinned.
+                     half and over float32 boundary and random patterns.  The complex64 and binary16
+                     contractions are pinned bit for bit by an independent NumPy restatement of the
+                     reference's executor (inplace_plan.py:80-135) with the engines' storage rounding and
+                     scale chain (tests/test_storage_restatement.py; misplaced roundings are shown to
+                     break it), and to binary16 accuracy by the dense state vector (C1).
                      Pinned by: the dense state vector of the Clifford+T circuit (statevector() below,
                      closed-form gate matrices) through block(); by gather_amplitude(), an independent
                      restatement of the other reference executor (dcs_rankwidth/scan_plan.py:161-195)
