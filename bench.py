@@ -50,6 +50,29 @@ def prefixes(count):
     return [(np.array([int(c) for c in r['input_data_bits']], np.uint8), r['draw_uniform']) for r in p['records']]
 
 
+# FP64 flops per live input entry of a step, by orbit code (kernel_opt/inplace_gpu.py:29-59): a complex
+# product is 6 flops, a complex sum 2.  0 grow: f0 a and f1 a; 1 mix: f0 a + f1 b per output; 2, 4: one
+# product; 3 replace: f0 (a + b) and f1 (a +- b) per pair; 5: f (a +- b) per pair; 6 shrink: two outputs
+# f0 (a + b) + f1 (e +- g) per quad; 7: W0 a + W1 b per pair.
+FLOPS_PER_INPUT = {0: 12, 1: 14, 2: 6, 3: 8, 4: 6, 5: 4, 6: 9, 7: 7}
+FP64_PEAK = 78.6e12          # MI355X FP64 vector, FLOP/s
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4   # wave64 VALU instructions per second: 4 SIMDs per CU, 4 cycles each
+
+
+def fused_flops_per_sample(engine):
+    """Algorithmic FP64 flops of the steps inside coset groups, per sample (live inputs only)."""
+    bp = engine.bp
+    kinds = bp.step_table()['kind']
+    st = bp.inplace_steps
+    total = 0
+    for kind, first, count, _ in engine.schedule():
+        if kind != 2:
+            continue
+        for i in range(first, first + count):
+            total += FLOPS_PER_INPUT[int(kinds[i])] << len(st[i].active_in)
+    return total << len(bp.slices)
+
+
 def fused_bytes_per_sample(engine, elem):
     """Algorithmic HBM bytes of the fused schedule per sample, per segment kind."""
     bp = engine.bp
@@ -165,6 +188,69 @@ def cpu_baseline(max_width=32):
             'seconds': dt}
 
 
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def config_lines(repeats=3):
+    """The smaller configurations measured directly on both sides: the 16q/d12/20T ring (all 2^16
+    amplitudes), the 32q/d40/120T ring (width 29, 2^16 blocks) and the 40q/d48/180T ring (width 31, 2^16
+    blocks), each on the GPU (complex64 storage, compiled schedule) and with the CPU oracle (OpenMP,
+    complex64 storage, FP64 arithmetic) over whole samples of the same seeded prefixes.  One JSON line per
+    configuration."""
+    from oracle import oracle
+    from dcs_amd.conditional_block import InplaceConditionalBlockGPU
+    from dcs_amd.sampling_protocol import make_protocol
+    from dcs_amd.serving import _synthetic
+    from dcs_amd.stabilizer import synthetic_instance
+    c1 = synthetic_instance(16, 12, 20, seed=0)
+    cfgs = [('16q/d12/20T ring, all 2^16 amplitudes', (c1['graph'], c1['unitaries'], 16, c1['natural_order'], [],
+                                                       list(range(16))), 1, 1)]
+    for n, d, t, lanes in ((32, 40, 120, 8), (40, 48, 180, 4)):
+        args, w = _synthetic(n, d, t, 0, 16)
+        cfgs.append((f'{n}q/d{d}/{t}T ring, width {w}, 2^16 block', args, 8, lanes))
+    threads = int(oracle.lib().oracle_threads())
+    for name, args, nprefix, lanes in cfgs:
+        eng = InplaceConditionalBlockGPU(*args)
+        bp = eng.bp
+        prot = make_protocol(0, nprefix, args[2], args[5], 1.0)
+        xs = [np.array([int(c) for c in r['input_data_bits']], np.uint8) for r in prot['records']]
+        us = [r['draw_uniform'] for r in prot['records']]
+        nbytes = bp.bytes_per_slice(8) * (1 << len(bp.slices))
+        eng.alloc(dtype=np.complex64, lanes=lanes, blocks=lanes)
+        for i in range(0, len(xs), lanes):                      # warm-up: first launches
+            eng.block_many(xs[i:i + lanes])
+        eng.ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            for i in range(0, len(xs), lanes):
+                vecs, _ = eng.block_many(xs[i:i + lanes])
+                for v, u in zip(vecs, us[i:i + lanes]):
+                    v.draw(u)
+        gpu_s = (time.perf_counter() - t0) / (repeats * len(xs))
+        first = np.asarray(eng.block(xs[0])[0])
+        eng.close()
+        t0 = time.perf_counter()
+        ref = oracle.block(bp, xs[0], storage=0)
+        oracle.draw(ref, us[0])
+        cpu_s = time.perf_counter() - t0
+        err = float(np.max(np.abs(first - ref)) / np.max(np.abs(ref)))
+        print(json.dumps({'config': name, 'width': bp.inplace_maxslots, 'block': 1 << bp.k,
+                          'algorithmic_bytes_per_sample': nbytes,
+                          'gpu': {'samples_per_s': 1 / gpu_s, 'ms_per_sample': 1e3 * gpu_s, 'GBps': nbytes / gpu_s / 1e9,
+                                  'prefixes': len(xs), 'lanes': lanes, 'storage': 'complex64', 'schedule': 'compiled'},
+                          'cpu': {'samples_per_s': 1 / cpu_s, 'ms_per_sample': 1e3 * cpu_s, 'GBps': nbytes / cpu_s / 1e9,
+                                  'threads': threads, 'nproc': os.cpu_count(), 'model': cpu_model(),
+                                  'kind': 'port', 'samples': 1},
+                          'gpu_over_cpu': cpu_s / gpu_s, 'max_rel_amp_diff_gpu_vs_cpu_c64': err}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -177,11 +263,16 @@ def main():
                     help='prefixes contracted together (frontiers held at once); falls back to 1 if they do not fit')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--configs', action='store_true',
+                    help='instead: the 16q, 32q and 40q ring configurations, GPU and CPU oracle, one line each')
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({'error': 'this benchmark runs on exactly one GPU; the sampler shards by sample '
                           'and needs no collectives', 'n_gpus': a.gpus}))
         sys.exit(2)
+    if a.configs:
+        config_lines()
+        return
     from dcs_amd import _native
     dev = _native.device_info(0)
     log('device', dev)
@@ -192,7 +283,7 @@ def main():
     copy_ms = ctx.stream_copy_ms(copy_bytes, 5)
     ctx.close()
     stream_GBps = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
-    log(f'stream copy: {stream_GBps:.1f} GB/s')
+    log(f'stream copy (best launch shape): {stream_GBps:.1f} GB/s')
 
     log('fast path: complex64, k2 plan, schedule', a.schedule)
     kw = {'lds_storage': not a.lds_fp64} if a.schedule == 'compiled' else {}
@@ -207,6 +298,7 @@ def main():
     bp = eng.bp
     unfused_bytes = bp.bytes_per_slice(8) * (1 << len(bp.slices))
     fb = fused_bytes_per_sample(eng, 8)
+    fused_flops = fused_flops_per_sample(eng)
     coset_ms = ms[2]
     coset_bytes = fb[2]
     achieved = coset_bytes / (coset_ms / 1e3) if coset_ms > 0 else None
@@ -238,16 +330,33 @@ def main():
         cpu = cpu_baseline()
         log('cpu', cpu)
 
-    traffic = None
+    traffic = valu = None
     kernel = 'dcs_group_*' if a.schedule == 'compiled' else 'k_coset'
-    tf = ROOT / 'profiles' / ('coset_groups_ibm64_k2_c64_traffic.json' if a.schedule == 'compiled'
-                              else 'k_coset_ibm64_k2_c64_traffic.json')
-    if tf.exists():
+    tf = ROOT / 'profiles' / 'coset_groups_ibm64_k2_c64_counters.json'
+    if a.schedule == 'compiled' and tf.exists():
         rec = json.loads(tf.read_text())
-        # counter-derived bytes belong to one schedule: use them only for the same launches and bytes
+        # counter readings belong to one schedule: use them only for the same launches and bytes
         if (rec.get('launches_per_sample') == float(launches[2]) and rec.get('lanes_per_launch', 1) == fast['lanes']
                 and rec.get('algorithmic_bytes_per_sample') == coset_bytes):
-            traffic = rec.get('hbm_bytes_per_launch')
+            traffic = rec.get('hbm_bytes_per_sample')
+            valu = rec.get('valu_instructions_per_sample')
+    # Three ceilings of the group kernels, each over their own time per sample: HBM bytes (algorithmic),
+    # FP64 flops (algorithmic), and VALU instruction issue (counted by SQ_INSTS_VALU in the committed
+    # counter profile).  `bound` names the resource closest to its ceiling.
+    sec = coset_ms / 1e3 if coset_ms > 0 else None
+    hbm_frac = achieved / HBM_PEAK if achieved else None
+    fp64_tflops = fused_flops / sec / 1e12 if sec else None
+    fp64_frac = fp64_tflops * 1e12 / FP64_PEAK if sec else None
+    valu_frac = valu / sec / VALU_ISSUE_PEAK if (valu and sec) else None
+    fracs = {'hbm': hbm_frac, 'fp64': fp64_frac, 'valu_issue': valu_frac}
+    bound = max((k for k in fracs if fracs[k] is not None), key=lambda k: fracs[k], default='hbm')
+    roofline = {'bound': bound, 'achieved': achieved / 1e9 if achieved else None, 'peak': HBM_PEAK / 1e9,
+                'unit': 'GB/s', 'frac': hbm_frac, 'traffic': traffic, 'traffic_unit': 'HBM bytes per sample',
+                'launches_per_sample': float(launches[2]), 'lanes_per_launch': fast['lanes'],
+                'kernel': kernel, 'bytes_per_sample': coset_bytes, 'ms_per_sample': coset_ms,
+                'fp64': {'achieved': fp64_tflops, 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s', 'frac': fp64_frac,
+                         'flops_per_sample': fused_flops},
+                'valu_issue': {'instructions_per_sample': valu, 'peak_per_s': VALU_ISSUE_PEAK, 'frac': valu_frac}}
     lat = np.array(fast['lat']) * 1e3
     line = {
         'metric': 'samples/s @64q 2^20 block: batched c64 vs plain c128, HBM GB/s; p50/p99 ms',
@@ -271,10 +380,7 @@ def main():
         'plain_c128': plain,
         'speedup_vs_plain_c128': (sps / plain['samples_per_s']) if plain else None,
         'parity_first_prefix': parity,
-        'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9 if achieved else None, 'peak': HBM_PEAK / 1e9,
-                     'unit': 'GB/s', 'frac': (achieved / HBM_PEAK) if achieved else None, 'traffic': traffic,
-                     'launches_per_sample': float(launches[2]), 'lanes_per_launch': fast['lanes'],
-                     'kernel': kernel, 'bytes_per_sample': coset_bytes, 'ms_per_sample': coset_ms},
+        'roofline': roofline,
         'cpu_baseline': cpu,
         'device': {k: dev[k] for k in ('device_name', 'arch', 'compute_units', 'total_bytes')},
     }

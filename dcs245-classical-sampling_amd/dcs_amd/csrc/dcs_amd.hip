@@ -746,8 +746,20 @@ __global__ void __launch_bounds__(64) k_interval_check(W w, const double *__rest
     if (!(lower <= u && u < upper)) d->status = kDrawIntervalDisagrees;
 }
 
-__global__ void k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, u64 n) {
-    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)blockDim.x * gridDim.x) dst[i] = src[i];
+// Bandwidth ceiling probe: a copy with U independent 16-byte loads in flight per thread, issued before
+// any of their stores, non-temporal (streamed past the caches), grid-stride over chunks of U * 256
+// entries; coalesced (consecutive threads take consecutive 16-byte entries).  n is a multiple of U * 256.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int U>
+__global__ void __launch_bounds__(256) k_stream_copy(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, u64 n) {
+    for (u64 base = (u64)blockIdx.x * (256 * U); base < n; base += (u64)gridDim.x * (256 * U)) {
+        u32x4 v[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) v[k] = __builtin_nontemporal_load(src + base + k * 256 + threadIdx.x);
+#pragma unroll
+        for (int k = 0; k < U; ++k) __builtin_nontemporal_store(v[k], dst + base + k * 256 + threadIdx.x);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2314,28 +2326,48 @@ int dcs_plan_profile(dcs_plan *p, int enable, double *ms, int64_t *launches) {
 }
 
 int dcs_stream_copy(dcs_context *ctx, uint64_t bytes, int reps, double *ms) {
-    if (!ctx || !ms || reps < 1 || bytes < 16 || (bytes & 15)) return fail(DCS_ERR_INVALID, "bad argument");
+    if (!ctx || !ms || reps < 1 || bytes < 16 * 2048 || (bytes % (16 * 2048)))
+        return fail(DCS_ERR_INVALID, "bytes must be a positive multiple of 32 KiB");
     HIPCHK(hipSetDevice(ctx->device));
-    double2 *a = nullptr, *b = nullptr;
+    u32x4 *a = nullptr, *b = nullptr;
     if (hipMalloc((void **)&a, bytes) != hipSuccess) return fail(DCS_ERR_NOMEM, "stream copy source");
     if (hipMalloc((void **)&b, bytes) != hipSuccess) {
         (void)hipFree(a);
         return fail(DCS_ERR_NOMEM, "stream copy destination");
     }
     const u64 n = bytes / 16;
-    const unsigned g = grid_for(n, ctx->cus, 64);
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipMemsetAsync(a, 0, bytes, ctx->stream));
-    k_stream_copy<<<g, kThreads, 0, ctx->stream>>>(a, b, n);
-    HIPCHK(hipEventRecord(e0, ctx->stream));
-    for (int r = 0; r < reps; ++r) k_stream_copy<<<g, kThreads, 0, ctx->stream>>>(a, b, n);
-    HIPCHK(hipEventRecord(e1, ctx->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    float t = 0.f;
-    HIPCHK(hipEventElapsedTime(&t, e0, e1));
-    *ms = t / reps;
+    // launch shapes: entries in flight per thread x resident workgroups per CU; the fastest is the ceiling
+    struct Shape {
+        int u;
+        unsigned per_cu;
+    };
+    const Shape shapes[] = {{2, 8}, {4, 8}, {8, 4}, {8, 8}, {4, 16}};
+    double best = 1e30;
+    for (const Shape &sh : shapes) {
+        const u64 chunks = n / (256ull * sh.u);
+        const unsigned g = (unsigned)std::max<u64>(1, std::min<u64>(chunks, (u64)ctx->cus * sh.per_cu));
+        auto launch = [&]() {
+            switch (sh.u) {
+                case 2: k_stream_copy<2><<<g, 256, 0, ctx->stream>>>(a, b, n); break;
+                case 4: k_stream_copy<4><<<g, 256, 0, ctx->stream>>>(a, b, n); break;
+                default: k_stream_copy<8><<<g, 256, 0, ctx->stream>>>(a, b, n); break;
+            }
+        };
+        launch();  // warm-up
+        HIPCHK(hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; ++r) launch();
+        HIPCHK(hipEventRecord(e1, ctx->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipGetLastError());
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        best = std::min(best, (double)t / reps);
+    }
+    *ms = best;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(a);

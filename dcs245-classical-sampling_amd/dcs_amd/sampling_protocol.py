@@ -15,130 +15,146 @@ import numpy as np
 
 
 def make_protocol(seed, count, n, free_vertices, mixture_weight=1.):
-    """Deterministic per-sample prefixes and uniforms from one PCG64 stream
-    (sampling_protocol.py:14-41): branch uniform, then either the closed bits (uint8) and the draw
-    uniform, or all n bits."""
+    """Per-sample prefixes and uniforms from one PCG64 stream (sampling_protocol.py:14-41).
+
+    The stream is consumed in a fixed order per sample -- one float for the branch, then either the
+    closed data bits as uint8 integers and one float for the draw (expensive branch), or all n bits
+    (uniform branch) -- so the same seed replays the same records."""
     free = [int(q) for q in free_vertices]
-    if len(set(free)) != len(free) or not all(0 <= q < n for q in free):
-        raise ValueError('Free vertices must be distinct data indices')
-    if not 0 <= mixture_weight <= 1 or count < 1:
-        raise ValueError('Invalid mixture weight or sample count')
-    closed = [q for q in range(n) if q not in set(free)]
-    rng = np.random.Generator(np.random.PCG64(seed))
-    records = []
-    for sample_id in range(count):
-        before = rng.bit_generator.state
-        branch_u = float(rng.random())
-        expensive = branch_u < mixture_weight
-        x = np.zeros(n, np.uint8)
-        if expensive:
-            x[closed] = rng.integers(0, 2, len(closed), dtype=np.uint8)
-            draw_u = float(rng.random())
+    if len(set(free)) != len(free) or any(not 0 <= q < n for q in free):
+        raise ValueError(f'free vertices {free} must be distinct data qubits in 0..{n - 1}')
+    if not 0 <= mixture_weight <= 1:
+        raise ValueError(f'mixture weight {mixture_weight!r} outside [0, 1]')
+    if count < 1:
+        raise ValueError(f'sample count {count!r} must be positive')
+    free_set = set(free)
+    closed = [q for q in range(n) if q not in free_set]
+    gen = np.random.Generator(np.random.PCG64(seed))
+    rows = []
+    for sid in range(count):
+        state0 = gen.bit_generator.state
+        u_branch = float(gen.random())
+        conditional = u_branch < mixture_weight
+        bits = np.zeros(n, np.uint8)
+        if conditional:
+            bits[closed] = gen.integers(0, 2, len(closed), dtype=np.uint8)
+            u_draw = float(gen.random())
         else:
-            x[:] = rng.integers(0, 2, n, dtype=np.uint8)
-            draw_u = None
-        records.append({'sample_id': sample_id, 'conditional_branch': expensive, 'branch_uniform': branch_u,
-                        'draw_uniform': draw_u, 'input_data_bits': ''.join(map(str, x)),
-                        'rng_before': before, 'rng_after': rng.bit_generator.state})
+            bits[:] = gen.integers(0, 2, n, dtype=np.uint8)
+            u_draw = None
+        rows.append({'sample_id': sid, 'conditional_branch': conditional, 'branch_uniform': u_branch,
+                     'draw_uniform': u_draw, 'input_data_bits': ''.join(str(int(b)) for b in bits),
+                     'rng_before': state0, 'rng_after': gen.bit_generator.state})
     return {'seed': int(seed), 'count': count, 'n': n, 'free_vertices': free, 'closed_vertices': closed,
             'mixture_weight': mixture_weight,
             'rng': 'NumPy PCG64; branch float64, uint8 vector bits, conditional float64',
-            'numpy_version': np.__version__, 'records': records}
+            'numpy_version': np.__version__, 'records': rows}
 
 
 def cdf_primary(amplitudes, uniform):
-    """The primary CDF draw only: (index, weights, normalised cdf) (sampling_protocol.py:56-65)."""
-    v = np.asarray(amplitudes, dtype=np.complex128)
-    if v.ndim != 1 or not len(v) or len(v) & (len(v) - 1):
-        raise ValueError('Expected a power-of-two one-dimensional amplitude vector')
-    if not np.isfinite(v).all() or not 0 <= uniform < 1:
-        raise ValueError('Nonfinite vector or invalid uniform')
-    weights = np.asarray(v.real * v.real + v.imag * v.imag, dtype='<f8', order='C')
-    cumulative = np.cumsum(weights, dtype=np.float64)
-    total = float(cumulative[-1])
-    if not total > 0 or not np.isfinite(total):
-        raise ArithmeticError('Numerically zero/nonfinite block: do not replace the query')
-    cumulative /= total
-    cumulative[-1] = 1.
-    index = int(np.searchsorted(cumulative, uniform, side='right'))
-    if index >= len(weights) or weights[index] <= 0:
-        raise ArithmeticError('CDF produced an invalid outcome')
-    return index, weights, cumulative, total
+    """The primary CDF draw only (sampling_protocol.py:56-65): (index, weights, normalised cdf, total).
+
+    weights = re^2 + im^2 in float64, cdf = numpy's sequential cumsum divided by its last entry, which is
+    then set to exactly 1, and the index is searchsorted(cdf, u, side='right')."""
+    amp = np.asarray(amplitudes, dtype=np.complex128)
+    size = amp.shape[0] if amp.ndim == 1 else 0
+    if size == 0 or size & (size - 1):
+        raise ValueError(f'amplitude vector of shape {amp.shape}: one dimension of power-of-two length expected')
+    if not 0 <= uniform < 1:
+        raise ValueError(f'uniform {uniform!r} outside [0, 1)')
+    if not np.isfinite(amp).all():
+        raise ValueError('amplitude vector holds a nonfinite entry')
+    w = np.asarray(amp.real * amp.real + amp.imag * amp.imag, dtype='<f8', order='C')
+    cdf = np.cumsum(w, dtype=np.float64)
+    mass = float(cdf[-1])
+    if not (np.isfinite(mass) and mass > 0):
+        raise ArithmeticError(f'block mass {mass!r}: the block is zero or nonfinite; the query is not replaced')
+    cdf /= mass
+    cdf[-1] = 1.
+    k = int(np.searchsorted(cdf, uniform, side='right'))
+    if k >= size or not w[k] > 0:
+        raise ArithmeticError(f'the CDF selected index {k}, which has no positive weight')
+    return k, w, cdf, mass
 
 
 def draw_checked(amplitudes, uniform):
-    """Draw once and check the whole CDF interval with math.fsum (sampling_protocol.py:44-88).
-
-    Never resamples: a disagreement between the numpy CDF and the accurately summed interval raises."""
-    v = np.asarray(amplitudes, dtype=np.complex128)
-    index, weights, cumulative, total_cdf = cdf_primary(v, uniform)
-    independent_mass = math.fsum(map(float, weights))
-    prefix_mass = math.fsum(map(float, weights[:index]))
-    lo, hi = prefix_mass / independent_mass, (prefix_mass + float(weights[index])) / independent_mass
-    if not lo <= uniform < hi:
-        raise ArithmeticError('Independent accurately summed CDF interval disagrees; do not resample')
-    grid_ends = np.ceil(cumulative * 2. ** 53)
-    grid_masses = np.diff(grid_ends, prepend=0.) / 2. ** 53
-    draw_tv = float(.5 * np.sum(np.abs(grid_masses - weights / independent_mass), dtype=np.float64))
-    return {'A_index': index, 'uniform': float(uniform), 'scaled_mass_cdf': total_cdf,
-            'scaled_mass_fsum': independent_mass, 'selected_scaled_weight': float(weights[index]),
-            'selected_conditional_probability': float(weights[index] / independent_mass),
-            'cdf_lower_numpy': float(cumulative[index - 1]) if index else 0.,
-            'cdf_upper_numpy': float(cumulative[index]), 'cdf_lower_fsum': lo, 'cdf_upper_fsum': hi,
-            'uniform_to_nearest_boundary': min(uniform - lo, hi - uniform),
+    """One draw, fail-closed (sampling_protocol.py:44-88): the primary CDF picks the index, then the
+    interval of that index is recomputed from exactly rounded sums (math.fsum) of the weights; if it does
+    not contain the uniform the draw raises instead of being repeated.  Also reports the total variation
+    between the sampler's finite 53-bit grid law and the normalised weights, and SHA-256 digests of the
+    weights and amplitudes."""
+    amp = np.asarray(amplitudes, dtype=np.complex128)
+    k, w, cdf, mass = cdf_primary(amp, uniform)
+    exact_mass = math.fsum(w)
+    below = math.fsum(w[:k])
+    lower, upper = below / exact_mass, (below + float(w[k])) / exact_mass
+    if not lower <= uniform < upper:
+        raise ArithmeticError(f'the exactly summed interval [{lower!r}, {upper!r}) of index {k} does not contain '
+                              f'the uniform {uniform!r}; the draw is refused, not repeated')
+    # random() yields multiples of 2^-53: bin i receives the grid points in [cdf[i-1], cdf[i])
+    grid = np.diff(np.ceil(cdf * 2. ** 53), prepend=0.) / 2. ** 53
+    tv = float(.5 * np.sum(np.abs(grid - w / exact_mass), dtype=np.float64))
+    return {'A_index': k, 'uniform': float(uniform), 'scaled_mass_cdf': mass,
+            'scaled_mass_fsum': exact_mass, 'selected_scaled_weight': float(w[k]),
+            'selected_conditional_probability': float(w[k] / exact_mass),
+            'cdf_lower_numpy': float(cdf[k - 1]) if k else 0., 'cdf_upper_numpy': float(cdf[k]),
+            'cdf_lower_fsum': lower, 'cdf_upper_fsum': upper,
+            'uniform_to_nearest_boundary': min(uniform - lower, upper - uniform),
             'independent_cdf_check_passed': True,
-            'finite_53bit_draw_TV_against_computed_normalized_weights': draw_tv,
-            'weights_sha256': hashlib.sha256(weights.tobytes()).hexdigest(),
-            'amplitudes_sha256': hashlib.sha256(v.astype('<c16', copy=False).tobytes()).hexdigest(),
-            'replay_scope': 'Full CDF checked in memory; hashes/interval retained. Independent '
-                            'regeneration requires rerunning the frozen block contraction.'}
+            'finite_53bit_draw_TV_against_computed_normalized_weights': tv,
+            'weights_sha256': hashlib.sha256(w.tobytes()).hexdigest(),
+            'amplitudes_sha256': hashlib.sha256(amp.astype('<c16', copy=False).tobytes()).hexdigest(),
+            'replay_scope': 'The whole CDF was checked in memory and only its digests and the drawn interval '
+                            'are kept; replaying the draw needs the block contracted again.'}
 
 
 def assemble_output(input_bits, free_vertices, A_index, ring=None):
-    """Bit j of A_index goes to free vertex free_vertices[j]; then the ring permutation and the
-    Qiskit (reversed) string (sampling_protocol.py:91-100)."""
-    bits = [int(b) for b in input_bits]
-    if not 0 <= A_index < 1 << len(free_vertices):
-        raise ValueError('Conditional index outside free-output range')
+    """Output bitstrings of one sample (sampling_protocol.py:91-100): bit j of A_index is written to free
+    data qubit free_vertices[j]; then the ring permutation maps logical qubit q to physical ring[q], and
+    the physical string is reversed (Qiskit order).  Returns (logical bits, Qiskit string)."""
+    logical = [int(b) for b in input_bits]
+    nfree = len(free_vertices)
+    if not 0 <= A_index < 1 << nfree:
+        raise ValueError(f'index {A_index} outside the {1 << nfree} outcomes of {nfree} free qubits')
     for j, q in enumerate(free_vertices):
-        bits[q] = (A_index >> j) & 1
-    if ring is None:
-        ring = list(range(len(bits)))
-    if sorted(ring) != list(range(len(bits))):
-        raise ValueError('Invalid ring permutation')
-    original = [0] * len(bits)
-    for q, actual in enumerate(ring):
-        original[actual] = bits[q]
-    return ''.join(map(str, bits)), ''.join(map(str, original[::-1]))
+        logical[q] = (A_index >> j) & 1
+    perm = list(range(len(logical))) if ring is None else list(ring)
+    if sorted(perm) != list(range(len(logical))):
+        raise ValueError(f'ring {perm} is not a permutation of the {len(logical)} qubits')
+    physical = [0] * len(logical)
+    for q, p in enumerate(perm):
+        physical[p] = logical[q]
+    return ''.join(map(str, logical)), ''.join(map(str, reversed(physical)))
 
 
 def actual_cdf_law_vector(amplitudes):
-    """Amplitudes whose probabilities equal the sampler's 53-bit grid law, source phases kept
-    (production/protocol_common.py:48-70)."""
-    a = np.asarray(amplitudes, dtype=np.complex128)
-    if a.ndim != 1 or not len(a) or not np.isfinite(a).all():
-        raise ValueError('Invalid vector')
-    weights = np.asarray(a.real * a.real + a.imag * a.imag, dtype='<f8', order='C')
-    cumulative = np.cumsum(weights, dtype=np.float64)
-    if not cumulative[-1] > 0:
-        raise ArithmeticError('Zero conditional vector; no silent replacement')
-    total = float(cumulative[-1])
-    cumulative /= total
-    cumulative[-1] = 1.
-    endpoints = np.ceil(cumulative * 2. ** 53).astype(np.uint64)
-    counts = np.diff(np.concatenate((np.zeros(1, np.uint64), endpoints)))
+    """The law the sampler really draws from, as amplitudes (production/protocol_common.py:48-70): outcome
+    i receives the multiples of 2^-53 in [cdf[i-1], cdf[i]), so its probability is that count over 2^53;
+    each amplitude keeps the phase of its source entry (phase 1 for a zero entry)."""
+    amp = np.asarray(amplitudes, dtype=np.complex128)
+    if amp.ndim != 1 or amp.size == 0 or not np.isfinite(amp).all():
+        raise ValueError(f'expected a nonempty, finite one-dimensional vector, got shape {amp.shape}')
+    w = np.asarray(amp.real * amp.real + amp.imag * amp.imag, dtype='<f8', order='C')
+    cdf = np.cumsum(w, dtype=np.float64)
+    mass = float(cdf[-1])
+    if not mass > 0:
+        raise ArithmeticError('the block is zero: there is no law to report and none is substituted')
+    cdf /= mass
+    cdf[-1] = 1.
+    ends = np.ceil(cdf * 2. ** 53).astype(np.uint64)
+    counts = np.diff(ends, prepend=np.uint64(0))
     if int(counts.sum(dtype=np.uint64)) != 1 << 53:
-        raise AssertionError('Grid counts do not normalize')
-    phases = np.ones(len(a), np.complex128)
-    nonzero = np.abs(a) > 0
-    phases[nonzero] = a[nonzero] / np.abs(a[nonzero])
-    law = phases * np.sqrt(counts.astype(np.float64) / 2. ** 53)
+        raise AssertionError(f'grid counts sum to {int(counts.sum(dtype=np.uint64))}, not 2^53')
+    phase = np.ones(amp.size, np.complex128)
+    nz = np.abs(amp) > 0
+    phase[nz] = amp[nz] / np.abs(amp[nz])
+    law = phase * np.sqrt(counts.astype(np.float64) / 2. ** 53)
 
-    def sha(b):
+    def digest(b):
         return hashlib.sha256(b).hexdigest()
     return law, {'grid_bits': 53, 'grid_total': 1 << 53,
-                 'grid_counts_sha256': sha(counts.astype('<u8', copy=False).tobytes()),
-                 'actual_law_vector_sha256': sha(law.astype('<c16', copy=False).tobytes()),
-                 'source_amplitudes_sha256': sha(a.astype('<c16', copy=False).tobytes()),
-                 'source_weights_sha256': sha(weights.tobytes()), 'source_cdf_total': total,
+                 'grid_counts_sha256': digest(counts.astype('<u8', copy=False).tobytes()),
+                 'actual_law_vector_sha256': digest(law.astype('<c16', copy=False).tobytes()),
+                 'source_amplitudes_sha256': digest(amp.astype('<c16', copy=False).tobytes()),
+                 'source_weights_sha256': digest(w.tobytes()), 'source_cdf_total': mass,
                  'law_norm_squared': float(np.vdot(law, law).real)}

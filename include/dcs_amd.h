@@ -241,8 +241,10 @@ int dcs_frontier_read(dcs_plan *plan, int lane, void *host_out);
  * index 0 step kernels, 1 resident runs, 2 coset groups, 3 block kernels, 4 draw. */
 int dcs_plan_profile(dcs_plan *plan, int enable, double *ms, int64_t *launches);
 
-/* Device bandwidth probe: copy `bytes` (multiple of 16) `reps` times with a streaming kernel and
- * return the average kernel time in ms. */
+/* Device bandwidth probe (the practical HBM ceiling; gpu_variants.py:29-37 is the reference's): copy
+ * `bytes` (a multiple of 32 KiB) `reps` times with a non-temporal streaming kernel under several launch
+ * shapes (2, 4 or 8 independent 16-byte loads in flight per thread, 4 to 16 workgroups per CU) and
+ * return the average kernel time in ms of the fastest shape. */
 int dcs_stream_copy(dcs_context *ctx, uint64_t bytes, int reps, double *ms);
 
 #ifdef __cplusplus
