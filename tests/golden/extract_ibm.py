@@ -10,7 +10,7 @@ list of inputs/circuit.qasm (one row per gate: code h=0 s=1 sx=2 t=3 cz=4, qubit
 the circuit file's SHA-256 and the frozen canonical-stabilizer hash of
 inputs/graph_provenance_validation.json.
 
-Usage: python extract_ibm.py /root/reference
+Usage (build container): python tests/golden/extract_ibm.py /root/reference
 """
 import hashlib
 import json
@@ -62,7 +62,8 @@ def main(ref):
         'circuit_gates': np.asarray(gates, np.int16),
         'provenance_json': np.frombuffer(json.dumps(provenance).encode(), np.uint8),
     }
-    dst = Path(__file__).resolve().parent / 'ibm_qat245_64q.npz'
+    # the workload file ships with the package (dcs_amd/data), which loads it at run time
+    dst = Path(__file__).resolve().parents[2] / 'dcs245-classical-sampling_amd' / 'dcs_amd' / 'data' / 'ibm_qat245_64q.npz'
     np.savez_compressed(dst, **out)
     print(dst, dst.stat().st_size)
 
