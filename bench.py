@@ -259,6 +259,7 @@ def main():
     ap.add_argument('--plain-samples', type=int, default=1)
     ap.add_argument('--schedule', default='compiled', choices=['compiled', 'fused', 'steps', 'steps_host'])
     ap.add_argument('--lds-fp64', action='store_true', help='compiled schedule: FP64 LDS tiles')
+    ap.add_argument('--lds-buffers', type=int, default=None, help='compiled schedule: tile pipeline (engine default if unset)')
     ap.add_argument('--lanes', type=int, default=2,
                     help='prefixes contracted together (frontiers held at once); falls back to 1 if they do not fit')
     ap.add_argument('--no-cpu', action='store_true')
@@ -287,6 +288,8 @@ def main():
 
     log('fast path: complex64, k2 plan, schedule', a.schedule)
     kw = {'lds_storage': not a.lds_fp64} if a.schedule == 'compiled' else {}
+    if a.lds_buffers is not None:
+        kw['lds_buffers'] = a.lds_buffers
     try:
         fast = run_path('inplace_k2_fp32', np.complex64, a.schedule, a.steps, a.warmup, profile=True, lanes=a.lanes, **kw)
     except MemoryError as exc:

@@ -273,7 +273,8 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    registers; 2 double-buffered tiles; 3 (default) two tiles, the next one loaded
                    straight from HBM into LDS (global_load_lds) during the current tile's passes, where
                    a group allows it (complex64 tiles in LDS, at least two lanes), else as 1; the same
-                   arithmetic as 1, so the same blocks bit for bit
+                   arithmetic as 1, so the same blocks bit for bit; 4 as 3 with the last register pass
+                   storing to HBM from registers and the scale folded into the group's first step
         graphs     replay each slice's launch sequence from a captured HIP graph (one per lane count)
         The legacy fp64_kernel_path argument is accepted and ignored, as in the reference."""
         dt = np.dtype(dtype)

@@ -155,8 +155,13 @@ inline u64 lane_dep_const(const GroupDev &g, uint32_t l) {
 // One register pass.  `T` names the current LDS buffer pointer in the generated code.
 // m0: coset coordinates whose address carries bit 0; the entry (t, l) then sits at LDS index
 // (t W + l) ^ parity(t & m0), a linear relabelling (zero outside the asynchronous-load mode).
+// fold: the group's power-of-two scale multiplies this pass's first step's coefficients (the scale is
+// exact, so every later value carries it and the stores need no multiply).  direct: this is the group's
+// last pass, and its results go from registers straight to HBM (rounded to storage, dead outputs
+// skipped, the maximum |stored|^2 kept) instead of back into the LDS tile; gb<m> / dlo<m> are the
+// per-thread address and liveness values set up by emit_direct_setup.
 inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView &v, const PassDev &ps, int threads,
-                      uint32_t m0 = 0) {
+                      uint32_t m0 = 0, bool fold = false, bool direct = false) {
     const GroupDev &g = *v.g;
     const int J = ps.nmask, NE = 1 << J, WB = g.wbits, D = g.d;
     const uint32_t norb = (1u << (D - J)) << WB;
@@ -196,7 +201,10 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
         const PStepDev &q = v.psteps[ps.first_pstep + s];
         emitf(o, "    { // step %d kind %d\n", q.step, q.kind);
         emitf(o, "      const Dyn &q = dl[%d];\n", q.step);
-        emitf(o, "      const cd f0{q.f0r, q.f0i};\n");
+        const bool sc = fold && s == 0;
+        if (sc) emitf(o, "      const double q_f1r = q.f1r * scale, q_f1i = q.f1i * scale;\n");
+        else emitf(o, "      const double q_f1r = q.f1r, q_f1i = q.f1i;\n");
+        emitf(o, sc ? "      const cd f0{q.f0r * scale, q.f0i * scale};\n" : "      const cd f0{q.f0r, q.f0i};\n");
         emitf(o, "      const int sg = (__popcll(tb & 0x%llxull) + q.ov) & 1;\n", (unsigned long long)q.beta);
         auto u = [&](int E) { return par32(TE[E] & q.bvec); };
         for (int m = 0; m < OPT; ++m) {
@@ -207,8 +215,8 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
             // +-f1 by flipping the sign bits (bit-identical to negation): no divergent branch around the
             // parameter loads
             emitf(o, "      const long long sm%d = (long long)((unsigned long long)sb%d << 63);\n", m, m);
-            emitf(o, "      const cd F%d{__longlong_as_double(__double_as_longlong(q.f1r) ^ sm%d), "
-                     "__longlong_as_double(__double_as_longlong(q.f1i) ^ sm%d)};\n", m, m, m);
+            emitf(o, "      const cd F%d{__longlong_as_double(__double_as_longlong(q_f1r) ^ sm%d), "
+                     "__longlong_as_double(__double_as_longlong(q_f1i) ^ sm%d)};\n", m, m, m);
             if (q.kind == 2 || q.kind == 7)
                 emitf(o, "      const cd W%d_0 = cadd(f0, F%d), W%d_1 = csub(f0, F%d);\n", m, m, m, m);
             if (q.kind == 4) emitf(o, "      const cd C%d_0 = q.y ? F%d : f0, C%d_1 = q.y ? cneg(F%d) : f0;\n", m, m, m, m);
@@ -324,11 +332,54 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
         }
         emitf(o, "    }\n");
     }
-    for (int m = 0; m < OPT; ++m)
-        for (int E = 0; E < NE; ++E)
-            emitf(o, "    T[b%d ^ %uu] = LP(x%d_%d);\n", m, (TE[E] << WB) ^ (uint32_t)par32(TE[E] & m0), m, E);
+    if (direct) {
+        // element E of orbit m is coset coordinate t_m ^ TE[E] (t_m with zeros at the pass pivots) and lane
+        // l_m: its address is tb ^ gb_m ^ addrv(TE[E]); it is dead after the group when its coordinate
+        // meets dead_out_t (statically when TE[E] does so at a pass pivot)
+        uint32_t pivm = 0;
+        for (int i = 0; i < J; ++i) pivm |= 1u << ps.piv[i];
+        const uint32_t DO = v.g->dead_out_t;
+        for (int m = 0; m < OPT; ++m)
+            for (int E = 0; E < NE; ++E) {
+                if (TE[E] & DO & pivm) continue;
+                const unsigned long long C = (unsigned long long)addr_of(v, TE[E]);
+                const bool rt = (DO & ~pivm) != 0;
+                if (rt) emitf(o, "    if (dlo%d == %uu) {\n", m, TE[E] & DO & ~pivm);
+                emitf(o, "    { const float2 r = LP(x%d_%d); E[tb ^ gb%d ^ 0x%llxull] = r; "
+                         "lmf = max_nan_f(lmf, r.x * r.x + r.y * r.y); }\n", m, E, m, C);
+                if (rt) emitf(o, "    }\n");
+            }
+    } else {
+        for (int m = 0; m < OPT; ++m)
+            for (int E = 0; E < NE; ++E)
+                emitf(o, "    T[b%d ^ %uu] = LP(x%d_%d);\n", m, (TE[E] << WB) ^ (uint32_t)par32(TE[E] & m0), m, E);
+    }
     if (norb < (uint32_t)threads) emitf(o, "    }\n");
     emitf(o, "    }\n");
+}
+
+// Per-thread constants of a group's last pass when it stores straight to HBM (emit_pass direct): for
+// each of the thread's orbits m, gb<m> = lane l_m ^ addrv(t_m) (the address part that does not change
+// from tile to tile; the lane deposit is the identity in the pipelines that use this) and dlo<m> =
+// t_m & dead_out_t outside the pass pivots.
+inline void emit_direct_setup(std::string &o, const GroupView &v, const PassDev &ps, int threads) {
+    const GroupDev &g = *v.g;
+    const int J = ps.nmask, WB = g.wbits, D = g.d;
+    const uint32_t norb = (1u << (D - J)) << WB;
+    const int OPT = norb >= (uint32_t)threads ? (int)(norb / threads) : 1;
+    uint32_t pivm = 0;
+    for (int i = 0; i < J; ++i) pivm |= 1u << ps.piv[i];
+    for (int m = 0; m < OPT; ++m) {
+        emitf(o, "  u64 gb%d;\n  uint32_t dlo%d;\n  {\n", m, m);
+        emitf(o, "    const uint32_t o = tid + %uu;\n", (uint32_t)(m * threads));
+        emitf(o, "    uint32_t t = o >> %d;\n", WB);
+        for (int i = 0; i < J; ++i) emitf(o, "    t = insert0(t, %d);\n", ps.piv[i]);
+        emitf(o, "    u64 a = (u64)(o & %uu);\n", (1u << WB) - 1);
+        for (int i = 0; i < D; ++i)
+            if (!((pivm >> i) & 1))
+                emitf(o, "    a ^= (0ull - (u64)((t >> %d) & 1u)) & 0x%llxull;\n", i, (unsigned long long)addr_of(v, 1u << i));
+        emitf(o, "    gb%d = a;\n    dlo%d = t & %uu;\n  }\n", m, m, g.dead_out_t & ~pivm);
+    }
 }
 
 // Workgroup size of a group's kernel: one thread per 8 tile entries (256 .. 1024 threads), so that
@@ -370,14 +421,16 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
     // even, swapped when the coset part of a carries bit 0 (the tile base never does, bit 0 being a lane
     // bit).  The LDS index permutation with m0 = the coordinates carrying bit 0 stores both
     // orientations where the 16-byte load puts them.
-    const bool dma = opt.lds_buffers == 3 && c64 && lds_is_storage && g.lane_identity && WB >= 1 && !guard &&
-                     PER >= 2;
+    const bool dma = (opt.lds_buffers == 3 || opt.lds_buffers == 4) && c64 && lds_is_storage && g.lane_identity &&
+                     WB >= 1 && !guard && PER >= 2;
+    // lds_buffers 4: the asynchronous-load pipeline whose last register pass stores to HBM from registers
+    const bool direct = dma && opt.lds_buffers == 4;
     uint32_t m0 = 0;
     if (dma)
         for (int i = 0; i < D; ++i)
             if (addr_of(v, 1u << i) & 1ull) m0 |= 1u << i;
     auto lidx = [&](uint32_t e) { return e ^ (uint32_t)par32((e >> WB) & m0); };
-    const bool single = opt.lds_buffers == 1 || (opt.lds_buffers == 3 && !dma);
+    const bool single = opt.lds_buffers == 1 || (opt.lds_buffers >= 3 && !dma);
     emitf(o, "  __shared__ Lt buf[%u];\n", single ? N : 2 * N);
     emitf(o, "  const u64 NT = %lluull;\n", (unsigned long long)g.ntiles);
     o += "  u64 ti = blockIdx.x;\n  if (ti >= NT) return;\n"
@@ -556,6 +609,36 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         o += "  auto dma_wait = [&]() { __builtin_amdgcn_s_waitcnt(0x0F70); };\n";
     }
     // pipeline
+    if (direct) {
+        // Each tile: issue the next tile's loads into U (free: the previous tile's last pass stored its
+        // results from registers), run the register passes on T, the last one storing to HBM; then wait
+        // for the loads and synchronise (every wave is done reading T before it becomes U).  The group's
+        // scale is folded into its first step, so stored values need no multiply.
+        emit_direct_setup(o, v, v.passes[g.npasses - 1], threads);
+        o += "  u64 tb = tile_base(ti);\n";
+        o += "  auto passes = [&](Lt *__restrict__ T, Lt *__restrict__ U, bool more, u64 tn) {\n"
+             "    if (more) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tile_base(tn)); __builtin_amdgcn_s_setprio(0); }\n";
+        for (int pi = 0; pi < g.npasses; ++pi) {
+            emit_pass(o, opt, v, v.passes[pi], threads, m0, pi == 0, pi + 1 == g.npasses);
+            if (pi + 1 < g.npasses) o += "    dcs_lds_barrier();\n";
+        }
+        o += "  };\n";
+        o += "  dma_fill(buf, tb);\n  dma_wait();\n  __syncthreads();\n"
+             "  int c = 0;\n"
+             "  for (;;) {\n";
+        emitf(o, "    Lt *T = buf + c * %u;\n    Lt *U = buf + (c ^ 1) * %u;\n", N, N);
+        o += "    const u64 tn = ti + gridDim.x;\n"
+             "    const bool more = tn < NT;\n"
+             "    passes(T, U, more, tn);\n"
+             "    if (!more) break;\n"
+             "    dma_wait();\n"
+             "    __syncthreads();\n"
+             "    c ^= 1;\n    ti = tn;\n    tb = tile_base(ti);\n"
+             "  }\n";
+        o += "  commit_max((double)lmf, &dmax[(u64)li * nunits + unit]);\n";
+        o += "}\n";
+        return o;
+    }
     if (dma) {
         // the tile body takes the two tiles as restrict pointers: the loads into U and the passes'
         // accesses to T get distinct alias scopes, so no wait for the loads is placed before the passes.

@@ -135,3 +135,60 @@ def best_rotation(graph, unitaries, n, natural_order, k, parity_operations=(), s
         if best is None or w < best[2]:
             best = (order, opn, w)
     return best
+
+
+def fit_width(graph, unitaries, n, natural_order, k, max_width, max_slices=6, stride=2, greedy_seeds=(0,),
+              cache_path=None):
+    """A plan of a ring circuit whose in-place frontier has at most 2^max_width entries, with the fewest
+    algorithmic bytes per sample among those tried: the narrowest rotation with the last k data qubits
+    open (best_rotation), then, if that is too wide, greedy parity slices of T references
+    (choose_parity_slices along the order the plan eliminates, open qubits last), keeping the shortest
+    prefix of slices that fits.  Each slice doubles the contractions per block, so fewer slices win
+    when both fit.  Returns dict(order, open_vertices, operations, width, slices, bytes_per_sample_c64)
+    or None when nothing tried fits.  With cache_path the result is kept in a JSON file under a digest of
+    the circuit and the arguments, so a server plans each circuit once."""
+    import hashlib
+    import json
+    from pathlib import Path
+    key = None
+    if cache_path is not None:
+        h = hashlib.sha256()
+        for a in (np.ascontiguousarray(graph, np.uint8), np.ascontiguousarray(unitaries, np.complex128),
+                  np.asarray(natural_order, np.int64)):
+            h.update(a.tobytes())
+        h.update(json.dumps([n, k, max_width, max_slices, stride, list(greedy_seeds)]).encode())
+        key = h.hexdigest()
+        path = Path(cache_path)
+        table = json.loads(path.read_text()) if path.exists() else {}
+        if key in table:
+            hit = table[key]
+            if hit is not None:
+                hit['operations'] = [tuple(op) for op in hit['operations']]
+            return hit
+    order, opn, width = best_rotation(graph, unitaries, n, natural_order, k, stride=stride)
+    best = None
+    if width <= max_width:
+        bp = BlockPlan(graph, unitaries, n, order, [], opn)
+        best = {'order': [int(v) for v in order], 'open_vertices': [int(v) for v in opn], 'operations': [],
+                'width': int(width), 'slices': 0, 'bytes_per_sample_c64': int(bp.bytes_per_slice(8))}
+    else:
+        A = np.asarray(graph, np.uint8)
+        N = len(A)
+        edges = [(int(a), int(b)) for a, b in zip(*np.nonzero(np.triu(A, 1)))]
+        opn_set = set(opn)
+        eliminated = [v for v in order if v not in opn_set] + [v for v in order if v in opn_set]
+        for seed in greedy_seeds:
+            ops, _ = choose_parity_slices(N, edges, eliminated, max_slices, seed=seed, allowed=set(range(n, N)))
+            for j in range(1, len(ops) + 1):
+                bp = BlockPlan(graph, unitaries, n, order, ops[:j], opn)
+                if bp.inplace_maxslots <= max_width:
+                    nbytes = int(bp.bytes_per_slice(8)) << j
+                    if best is None or nbytes < best['bytes_per_sample_c64']:
+                        best = {'order': [int(v) for v in order], 'open_vertices': [int(v) for v in opn],
+                                'operations': [tuple(op) for op in ops[:j]], 'width': int(bp.inplace_maxslots),
+                                'slices': j, 'bytes_per_sample_c64': nbytes}
+                    break
+    if key is not None:
+        table[key] = best
+        Path(cache_path).write_text(json.dumps(table, indent=1) + '\n')
+    return best

@@ -118,7 +118,10 @@ typedef struct {
                                    default when 0) or 1 (the next tile waits in registers); 3: two
                                    tiles, the next one loaded from HBM straight into LDS
                                    (global_load_lds) during the current one's passes, in groups whose
-                                   complex64 tiles allow it (else as 1); same results as 1 */
+                                   complex64 tiles allow it (else as 1); same results as 1;
+                                   4: as 3, but the last register pass stores its results to HBM
+                                   from registers and the group's scale is folded into its first
+                                   step (no LDS write-back, loads issued a whole tile ahead) */
 } dcs_plan_desc;
 
 typedef struct {

@@ -29,6 +29,8 @@ EMU = Path(__file__).resolve().parent / 'emu'
 # (storage, tile_log2, fuse_max_dim, group_policy, lds_buffers, exact_rounding, gridcap)
 CASES = [
     (0, 12, 8, 1, 3, True, 1),    # complex64, asynchronous loads, one workgroup walks every tile
+    (0, 12, 8, 1, 4, True, 1),    # complex64, asynchronous loads, last pass stores from registers
+    (0, 12, 8, 2, 4, True, 3),    # the same with three workgroups and bytes-and-passes groups
     (0, 11, 6, 0, 2, True, 3),    # complex64, two register-fed LDS tiles on 2^11 tiles, greedy groups
     (1, 12, 8, 1, 1, False, 2),   # complex128, one LDS tile
     (2, 12, 8, 2, 3, True, 2),    # binary16 storage (one-tile pipeline), bytes-and-passes groups
@@ -60,8 +62,10 @@ def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, p
                                       fuse_max_dim=dim, resident_bits=0, tile_log2=tl, exact_rounding=exact,
                                       pass_max_dim=3, lds_storage=True, group_policy=policy, lds_buffers=buffers)
     assert segs and len(set(s[0] for s in segs)) == len(segs)
-    if buffers == 3 and storage == 0:
+    if buffers >= 3 and storage == 0:
         assert 'global_load_lds' in src        # the asynchronous pipeline is what runs here
+    if buffers == 4 and storage == 0:
+        assert 'E[tb ^ gb0 ^ ' in src          # and its last pass stores from registers
     (tmp_path / 'kernels.inc').write_text(adapt(src))
     names = sorted({int(m) for m in re.findall(r'\bdcs_group_(\d+)\(St ', src)})
     assert names == list(range(len(names)))

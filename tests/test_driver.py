@@ -138,3 +138,52 @@ def test_verifier_catches_changed_reference_vector(tmp_path, small):
     np.save(path, v)
     codes = {f['code'] for f in verify_worker_dir(out, workload=small[0])['failures']}
     assert 'vector_payload_hash' in codes and 'witness_reference_scaled_mass' in codes
+
+
+def test_run_verification_over_every_worker(tmp_path, small):
+    """Two workers of a protocol whose audit ids come from an audit seed: the run-level checks pass over
+    both directories (selection replayed, every id once, one anchor hash, homogeneous hardware) and the
+    certificate pools every audit; a missing worker or a changed selection fails them."""
+    import copy
+    import hashlib
+    from dcs_amd.driver import replay_audit_selection, verify_run
+    workload, make_engine = small
+    w = copy.deepcopy(workload)
+    prot = w['protocol']
+    prot['audit'] = {'seed': 99, 'per_worker': 2, 'alpha': 0.05}
+    prot['audit']['ids_by_worker'] = replay_audit_selection(prot['audit'], 9, 2)
+    prot['audit']['ids'] = sorted(i for v in prot['audit']['ids_by_worker'].values() for i in v)
+    x = np.zeros(12, np.uint8)
+    anchor = hashlib.sha256(make_engine('gen', np.complex64).block(x)[0].astype('<c16').tobytes()).hexdigest()
+    dirs, results = [], {}
+    for wid in (0, 1):
+        out = tmp_path / f'worker_{wid}'
+        run_worker(out, wid, workload=w, make_engine=make_engine, hardware=False, anchor_expected_sha256=anchor,
+                   log=lambda *a, **k: None)
+        (out / 'hardware.json').write_text(json.dumps({'device_name': 'cpu', 'arch': 'x', 'numpy': '2',
+                                                       'hip_runtime_version': 1, 'hip_driver_version': 1,
+                                                       'library_sha256': 'abc'}))
+        dirs.append(out)
+        results[wid] = verify_worker_dir(out, workload=w)
+    res = verify_run(dirs, results, workload=w)
+    assert res['passed'], res['failures'][:5]
+    assert res['sample_count'] == 9 and res['audit_count'] == 4 and res['quality']['audit_count'] == 4
+    assert res['anchor_sha256'] == anchor
+    bad = verify_run(dirs[:1], {0: results[0]}, workload=w)
+    assert {'missing_worker_directories', 'all_sample_ids_exactly_once'} <= {f['code'] for f in bad['failures']}
+    w2 = copy.deepcopy(w)
+    w2['protocol']['audit']['seed'] = 100
+    assert 'audit_selection_replay' in {f['code'] for f in verify_run(dirs, results, workload=w2)['failures']}
+
+
+def test_compare_draws_counts_ties(tmp_path, small):
+    from dcs_amd.driver import compare_draws
+    out, _ = run(tmp_path, small)
+    rows = [json.loads(l) for l in (out / 'samples.jsonl').read_text().splitlines()]
+    ref = {str(r['sample_id']): {'A_index': r['draw']['A_index'], 'margin': 1.0} for r in rows}
+    assert compare_draws([out], ref) == {'compared': 5, 'identical': 5, 'ties': 0, 'differ': 0, 'differing_ids': [],
+                                         'tie_margin': 1e-7}
+    ref['0'] = {'A_index': rows[0]['draw']['A_index'] ^ 1, 'margin': 1e-9}
+    ref['2'] = {'A_index': rows[1]['draw']['A_index'] ^ 1, 'margin': 0.5}
+    res = compare_draws([out], ref)
+    assert (res['identical'], res['ties'], res['differ'], res['differing_ids']) == (3, 1, 1, [2])

@@ -268,3 +268,27 @@ def test_greedy_slicing_reproduces_the_frozen_ibm_k5_operations(ibm64):
     want = [tuple(None if x is None else (x if isinstance(x, str) else int(x)) for x in op)
             for op in ibm64['plans']['canonical_k5_fp64']['operations']]
     assert ops == want
+
+
+def test_fit_width_slices_to_the_budget_and_keeps_the_block(tmp_path):
+    """A 16-qubit ring whose best rotation has width 7: fit_width reaches width 5 with greedy slices of T
+    references, the sliced block equals the unsliced one up to one global phase (oracle, complex128),
+    and a second call is served from the plan cache."""
+    from dcs_amd.width_planner import fit_width
+    n, k = 16, 4
+    inst = synthetic_instance(n, 14, 28, seed=2)
+    order, opn, w = best_rotation(inst['graph'], inst['unitaries'], n, inst['natural_order'], k, stride=2)
+    assert w == 7
+    cache = tmp_path / 'plans.json'
+    r = fit_width(inst['graph'], inst['unitaries'], n, inst['natural_order'], k, 5, max_slices=4, cache_path=cache)
+    assert r['width'] <= 5 and r['slices'] >= 1 and all(op[1] >= n for op in r['operations'])
+    bp = BlockPlan(inst['graph'], inst['unitaries'], n, r['order'], r['operations'], r['open_vertices'])
+    assert bp.inplace_maxslots == r['width']
+    x = np.random.default_rng(4).integers(0, 2, n, dtype=np.uint8)
+    full = oracle.block(BlockPlan(inst['graph'], inst['unitaries'], n, r['order'], [], r['open_vertices']), x)
+    sl = oracle.block(bp, x)
+    i = int(np.argmax(np.abs(full)))
+    assert np.max(np.abs(sl - (sl[i] / full[i]) * full)) / np.max(np.abs(full)) < 1e-12
+    again = fit_width(inst['graph'], inst['unitaries'], n, inst['natural_order'], k, 5, max_slices=4, cache_path=cache)
+    assert again == r
+    assert fit_width(inst['graph'], inst['unitaries'], n, inst['natural_order'], k, 3, max_slices=2) is None
