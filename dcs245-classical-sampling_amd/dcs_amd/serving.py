@@ -403,45 +403,73 @@ def _synthetic(n, depth, t, seed, k):
     return (inst['graph'], inst['unitaries'], n, order, [], opn), width
 
 
+RING_PLANS = __import__('pathlib').Path(__file__).resolve().parent / 'data' / 'ring_plans.json'
+
+
+def c5_mix(srv, plan_cache=RING_PLANS, with_2_22=True, log=None):
+    """Add the serving mix of ring Clifford+T circuits to a server (nothing is allocated until a request
+    is admitted): 40q/d48/180T with 2^16 blocks (best ring rotation, width 31), 52q/d60/250T with 2^18
+    blocks (circuit seed 1: best rotation plus greedy parity slices of T references down to width 33,
+    8 slices per block) and the 64-qubit IBM QAT #245 graph with 2^20 and 2^22 blocks (plan
+    inplace_k2_fp32, width 34; the 2^22 circuit opens the two next data qubits of the order), weighted
+    3 : 2 : 1 (the 64-qubit weight split between the two block sizes).  Returns the Poisson mix table."""
+    from .stabilizer import synthetic_instance
+    from .width_planner import fit_width
+    from .workloads import load_ibm64
+    kw = dict(schedule='compiled', lds_storage=True)
+    args40, w40 = _synthetic(40, 48, 180, 0, 16)
+    srv.add('40q/d48/180T', *args40, lanes=4, max_batch=4, **kw)
+    inst = synthetic_instance(52, 60, 250, 1)
+    fit = fit_width(inst['graph'], inst['unitaries'], 52, inst['natural_order'], 18, 33, max_slices=4,
+                    cache_path=plan_cache)
+    srv.add('52q/d60/250T', inst['graph'], inst['unitaries'], 52, fit['order'], fit['operations'],
+            fit['open_vertices'], lanes=1, max_batch=2, **kw)
+    w = load_ibm64()
+    ops = w['plans']['inplace_k2_fp32']['operations']
+    opn20 = [int(v) for v in w['open_vertices']]
+    srv.add('64q/2^20', w['graph'], w['unitaries'], 64, w['order'], ops, opn20, lanes=1, max_batch=2, **kw)
+    mix = {'40q/d48/180T': (3, 40, args40[5]), '52q/d60/250T': (2, 52, fit['open_vertices']),
+           '64q/2^20': (0.5 if with_2_22 else 1, 64, opn20)}
+    if with_2_22:
+        extra = [int(v) for v in w['order'] if v < 64 and int(v) not in set(opn20)][::-1][:2]
+        srv.add('64q/2^22', w['graph'], w['unitaries'], 64, w['order'], ops, opn20 + extra, lanes=1, max_batch=2,
+                **kw)
+        mix['64q/2^22'] = (0.5, 64, opn20 + extra)
+    if log:
+        log(f'widths: 40q {w40}, 52q {fit["width"]} with {fit["slices"]} slices, 64q 34')
+    return mix, {'40q/d48/180T': w40, '52q/d60/250T': fit['width'], '52q_slices': fit['slices'], '64q': 34}
+
+
 def main(argv=None):
     import argparse
-    ap = argparse.ArgumentParser(description='Serving benchmark: Poisson mix of resident circuits on one GPU')
+    ap = argparse.ArgumentParser(description='Serving benchmark: Poisson mix of ring circuits on one GPU')
     ap.add_argument('--requests', type=int, default=36)
-    ap.add_argument('--rate', type=float, default=4.0, help='mean arrivals per second')
+    ap.add_argument('--rate', type=float, default=2.0, help='mean arrivals per second')
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--no-ibm', action='store_true')
-    ap.add_argument('--realtime', action='store_true')
-    ap.add_argument('--concurrent', action='store_true', help='one worker thread per circuit (real-time arrivals)')
+    ap.add_argument('--no-2-22', action='store_true', help='64-qubit requests with 2^20 blocks only')
+    ap.add_argument('--sequential', action='store_true', help='one loop instead of one worker thread per circuit')
     a = ap.parse_args(argv)
-    from .workloads import load_ibm64
+    log = lambda *m: print(*m, file=__import__('sys').stderr, flush=True)
     srv = Server()
-    mix = {}
     t0 = time.perf_counter()
-    args32, w32 = _synthetic(32, 40, 120, 0, 16)
-    srv.register('32q/d40/120T', *args32, lanes=8, max_batch=4, schedule='compiled', lds_storage=True)
-    mix['32q/d40/120T'] = (3, 32, args32[5])
-    args40, w40 = _synthetic(40, 48, 180, 0, 16)
-    srv.register('40q/d48/180T', *args40, lanes=4, max_batch=4, schedule='compiled', lds_storage=True)
-    mix['40q/d48/180T'] = (2, 40, args40[5])
-    if not a.no_ibm:
-        w = load_ibm64()
-        srv.register('ibm64', w['graph'], w['unitaries'], 64, w['order'], w['plans']['inplace_k2_fp32']['operations'],
-                     w['open_vertices'], lanes=1, max_batch=2, schedule='compiled', lds_storage=True)
-        mix['ibm64'] = (1, 64, w['open_vertices'])
+    mix, widths = c5_mix(srv, with_2_22=not a.no_2_22, log=log)
+    # warm-up: one request per circuit, alone (admission, hipRTC compile, first launches), outside the stats
+    for j, name in enumerate(mix):
+        srv.serve([Request(-1 - j, name, np.zeros(mix[name][1], np.uint8), 0.5, 0.0)], realtime=False)
+        log(f'warm {name}: resident {list(srv.circuits)}')
     setup = time.perf_counter() - t0
-    # one warm-up request per circuit (first launches, hipRTC modules already compiled at registration)
-    warm = [Request(-1 - j, name, np.zeros(mix[name][1], np.uint8), 0.5, 0.0) for j, name in enumerate(mix)]
-    srv.serve(warm, realtime=False)
+    evictions0 = srv.evictions
     reqs = poisson_workload(mix, a.rate, a.requests, a.seed)
-    srv.serve(reqs, realtime=a.realtime or a.concurrent, concurrent=a.concurrent)
+    t1 = time.perf_counter()
+    srv.serve(reqs, realtime=True, concurrent=not a.sequential)
+    span = time.perf_counter() - t1
     stats = latency_stats(reqs)
-    line = {'metric': 'serving: continuous batching over resident circuits, p50/p99 ms per circuit',
-            'requests': a.requests, 'rate_per_s': a.rate, 'seed': a.seed, 'realtime': a.realtime or a.concurrent,
-            'concurrent': a.concurrent,
-            'setup_s': setup, 'hbm_used_bytes': srv.used, 'hbm_budget_bytes': srv.budget,
-            'circuits': {c.name: {'lanes': c.lanes, 'max_batch': c.max_batch, 'device_bytes': c.nbytes,
-                                  'rmax': c.engine.inplace_rmax, 'served': c.served} for c in srv.circuits.values()},
-            'widths': {'32q/d40/120T': w32, '40q/d48/180T': w40}, 'stats': stats}
+    line = {'metric': 'serving: continuous batching with admission by HBM footprint, p50/p99 ms per circuit',
+            'requests': a.requests, 'rate_per_s': a.rate, 'seed': a.seed, 'concurrent': not a.sequential,
+            'setup_s': setup, 'serve_s': span, 'samples_per_s': len(reqs) / span,
+            'evictions_during_serving': srv.evictions - evictions0, 'hbm_budget_bytes': srv.budget,
+            'admissions': {name: spec.admissions for name, spec in srv.catalog.items()},
+            'widths': widths, 'stats': stats}
     print(json.dumps(line), flush=True)
     srv.close()
 
