@@ -7,7 +7,8 @@ every slice contraction, the coherent slice sum, the Walsh-Hadamard / phase / Cl
 device CDF draw.  Inputs (graph, plan, frontier buffer) are resident before the timed region.
 
 Fast path (value): complex64 storage, plan inplace_k2_fp32 (4 slices, rmax 34, 128 GiB frontier per
-lane; two prefixes at a time by default, their slices sharing each launch),
+lane; two prefixes at a time by default, their slices sharing each launch; each slice pair's launch
+sequence replayed from a captured HIP graph),
 compiled schedule (LDS-resident runs + one generated kernel per coset group, complex64 LDS tiles
 filled by global-to-LDS loads; coset groups of dimension <= 8 on 2^12-entry tiles with the
 fewest-HBM-bytes group boundaries).
@@ -260,6 +261,8 @@ def main():
     ap.add_argument('--schedule', default='compiled', choices=['compiled', 'fused', 'steps', 'steps_host'])
     ap.add_argument('--lds-fp64', action='store_true', help='compiled schedule: FP64 LDS tiles')
     ap.add_argument('--lds-buffers', type=int, default=None, help='compiled schedule: tile pipeline (engine default if unset)')
+    ap.add_argument('--no-graphs', action='store_true',
+                    help='launch every kernel of a slice directly instead of replaying its captured HIP graph')
     ap.add_argument('--lanes', type=int, default=2,
                     help='prefixes contracted together (frontiers held at once); falls back to 1 if they do not fit')
     ap.add_argument('--no-cpu', action='store_true')
@@ -290,6 +293,8 @@ def main():
     kw = {'lds_storage': not a.lds_fp64} if a.schedule == 'compiled' else {}
     if a.lds_buffers is not None:
         kw['lds_buffers'] = a.lds_buffers
+    if a.schedule != 'steps_host':
+        kw['graphs'] = not a.no_graphs
     try:
         fast = run_path('inplace_k2_fp32', np.complex64, a.schedule, a.steps, a.warmup, profile=True, lanes=a.lanes, **kw)
     except MemoryError as exc:
@@ -372,7 +377,7 @@ def main():
                                '(4 parity slices, rmax 34, 128 GiB c64 frontier), 2^20 block, 44-bit prefixes',
                    'schedule': a.schedule, 'lds_tiles': ('fp64' if a.lds_fp64 else 'storage') if a.schedule == 'compiled' else None,
                    'fusion': {k: fast['options'][k] for k in ('fuse_max_dim', 'tile_log2', 'pass_max_dim', 'group_policy',
-                                                               'lds_buffers')},
+                                                               'lds_buffers', 'graphs')},
                    'batch': fast['lanes'],
                    'block': 1 << 20},
         'setup_s': fast['setup'], 'fast_samples_run': fast['samples_run'],
