@@ -252,6 +252,81 @@ def config_lines(repeats=3):
                           'gpu_over_cpu': cpu_s / gpu_s, 'max_rel_amp_diff_gpu_vs_cpu_c64': err}), flush=True)
 
 
+BATCH_MIX = [(16, 12, 20, 0, 8), (20, 16, 40, 1, 10), (24, 24, 80, 3, 8), (28, 32, 96, 1, 10),
+             (32, 30, 100, 0, 12), (24, 20, 60, 2, 10), (28, 24, 80, 2, 10), (32, 28, 90, 1, 12)]
+
+
+def batch_line(prefixes=4, repeats=5):
+    """Small circuits of uneven frontier widths (seeded ring circuits, widths 7 to 19, 2^8 - 2^12
+    blocks), `prefixes` requests each: one JSON line with the throughput of block_batch (every circuit's
+    segments in shared launches, generic kernels) against the same engines one request at a time, and
+    against each circuit's own compiled engine with its requests batched in lanes."""
+    from dcs_amd import _native
+    from dcs_amd.block_plan import BlockPlan
+    from dcs_amd.conditional_block import InplaceConditionalBlockGPU, block_batch
+    from dcs_amd.sampling_protocol import make_protocol
+    from dcs_amd.stabilizer import synthetic_instance
+    from dcs_amd.width_planner import best_rotation
+    cases = []
+    for n, d, t, seed, k in BATCH_MIX:
+        inst = synthetic_instance(n, d, t, seed)
+        order, opn, _ = best_rotation(inst['graph'], inst['unitaries'], n, inst['natural_order'], k, stride=2)
+        args = (inst['graph'], inst['unitaries'], n, order, [], opn)
+        prot = make_protocol(seed, prefixes, n, opn, 1.0)
+        xs = [np.array([int(c) for c in r['input_data_bits']], np.uint8) for r in prot['records']]
+        cases.append((args, xs, BlockPlan(*args).inplace_maxslots))
+    ctx = _native.Context(0)
+    fused = []
+    for args, xs, _ in cases:
+        e = InplaceConditionalBlockGPU(*args)
+        e.alloc(dtype=np.complex64, schedule='fused', lanes=prefixes, blocks=prefixes + 1, context=ctx)
+        fused.append(e)
+    requests = [(e, x, b) for e, (_, xs, _) in zip(fused, cases) for b, x in enumerate(xs)]
+    nreq = len(requests)
+
+    def timed(fn, sync):
+        fn()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            fn()
+        sync()
+        return (time.perf_counter() - t0) / repeats
+
+    meta = {}
+
+    def batched():
+        meta.update(block_batch(requests)[1])
+
+    t_batch = timed(batched, ctx.synchronize)
+    first = np.asarray(block_batch(requests[:1])[0][0])
+    t_alone = timed(lambda: [e.block(x, block_index=b) for e, x, b in requests], ctx.synchronize)
+    same = bool(np.array_equal(np.asarray(requests[0][0].block(requests[0][1], block_index=prefixes)[0]), first))
+    segments = sum(len(e.schedule()) for e in fused)
+    for e in fused:
+        e.close()
+    ctx.close()
+    compiled = []
+    for args, xs, _ in cases:
+        e = InplaceConditionalBlockGPU(*args)
+        e.alloc(dtype=np.complex64, lanes=prefixes, blocks=prefixes)
+        compiled.append((e, xs))
+    t_comp = timed(lambda: [e.block_many(xs) for e, xs in compiled],
+                   lambda: [e.ctx.synchronize() for e, _ in compiled])
+    for e, _ in compiled:
+        e.close()
+    print(json.dumps({'config': 'batch of small circuits of uneven width',
+                      'circuits': [f'{n}q/d{d}/{t}T seed {s}, width {w}, 2^{k} block'
+                                   for (n, d, t, s, k), (_, _, w) in zip(BATCH_MIX, cases)],
+                      'requests': nreq, 'storage': 'complex64',
+                      'block_batch': {'samples_per_s': nreq / t_batch, 'ms_per_round': 1e3 * t_batch,
+                                      'launches': meta.get('launches'), 'rounds': meta.get('rounds')},
+                      'fused_one_at_a_time': {'samples_per_s': nreq / t_alone, 'ms': 1e3 * t_alone,
+                                              'segment_launches': segments * prefixes},
+                      'compiled_per_circuit_lanes': {'samples_per_s': nreq / t_comp, 'ms': 1e3 * t_comp},
+                      'batch_equals_alone_first_request': same}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -269,6 +344,8 @@ def main():
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--configs', action='store_true',
                     help='instead: the 16q, 32q and 40q ring configurations, GPU and CPU oracle, one line each')
+    ap.add_argument('--batch-mix', action='store_true',
+                    help='instead: eight small circuits of uneven width, shared launches against one at a time')
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({'error': 'this benchmark runs on exactly one GPU; the sampler shards by sample '
@@ -276,6 +353,9 @@ def main():
         sys.exit(2)
     if a.configs:
         config_lines()
+        return
+    if a.batch_mix:
+        batch_line()
         return
     from dcs_amd import _native
     dev = _native.device_info(0)

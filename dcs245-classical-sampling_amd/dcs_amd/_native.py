@@ -24,7 +24,8 @@ EXPORTED = (
     'dcs_plan_units',
     'dcs_plan_schedule', 'dcs_contract', 'dcs_block_zero', 'dcs_block_accumulate', 'dcs_block_transform',
     'dcs_block_read', 'dcs_block_device_pointer', 'dcs_draw', 'dcs_frontier_read', 'dcs_plan_profile',
-    'dcs_stream_copy', 'dcs_draw_noalloc')
+    'dcs_stream_copy', 'dcs_draw_noalloc', 'dcs_contract_batch')
+BATCH_MAX = 16   # DCS_BATCH_MAX
 
 
 class DeviceInfo(C.Structure):
@@ -44,6 +45,12 @@ class PlanDesc(C.Structure):
                 ('fuse_max_dim', C.c_int32), ('resident_bits', C.c_int32), ('tile_log2', C.c_int32),
                 ('exact_rounding', C.c_int32), ('pass_max_dim', C.c_int32), ('lds_storage', C.c_int32),
                 ('group_policy', C.c_int32), ('lds_buffers', C.c_int32)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [('plan', C.c_void_p), ('nlanes', C.c_int), ('lanes', C.POINTER(C.c_int32)),
+                ('coeffs', C.POINTER(C.c_double)), ('ov', C.POINTER(C.c_int32)), ('y', C.POINTER(C.c_int32)),
+                ('unit_m2', C.POINTER(C.c_double))]
 
 
 class GatherDesc(C.Structure):
@@ -108,6 +115,7 @@ def load():
         'dcs_plan_use_graphs': (C.c_int, [vp, C.c_int]),
         'dcs_stream_copy': (C.c_int, [vp, u64, C.c_int, P(dbl)]),
         'dcs_draw_noalloc': (C.c_int, [vp, vp, C.c_int, dbl, P(DrawResult)]),
+        'dcs_contract_batch': (C.c_int, [C.c_int, P(BatchItem), P(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -140,6 +148,31 @@ def draw_dict(r):
             'cdf_lower_numpy': r.cdf_lower, 'cdf_upper_numpy': r.cdf_upper, 'cdf_margin': r.margin,
             'scaled_mass_compensated': r.mass_check, 'cdf_lower_compensated': r.cdf_lower_check,
             'cdf_upper_compensated': r.cdf_upper_check, 'independent_cdf_check_passed': True}
+
+
+def contract_batch(items):
+    """dcs_contract_batch: items = [(plan, lanes, coeffs, ov, y)] of distinct plans on one context, laid out
+    as for Plan.contract.  Runs their fused schedules together (each round's resident runs and coset
+    groups in one launch each); returns ([float64[nlanes, nunits] unit maxima per item], launches)."""
+    if not items:
+        return [], 0
+    arr = (BatchItem * len(items))()
+    keep, outs = [], []
+    for b, (plan, lanes, coeffs, ov, y) in zip(arr, items):
+        lanes = np.ascontiguousarray(lanes, dtype=np.int32)
+        nl = len(lanes)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(nl, plan.nsteps, 4)
+        ov = np.ascontiguousarray(ov, dtype=np.int32).reshape(nl, plan.nsteps)
+        y = np.ascontiguousarray(y, dtype=np.int32).reshape(nl, plan.nsteps)
+        out = np.zeros((nl, plan.units(MODE_FUSED)), np.float64)
+        keep.append((lanes, coeffs, ov, y))
+        outs.append(out)
+        b.plan, b.nlanes = plan.handle.value, nl
+        b.lanes, b.coeffs = ptr(lanes, C.c_int32), ptr(coeffs, C.c_double)
+        b.ov, b.y, b.unit_m2 = ptr(ov, C.c_int32), ptr(y, C.c_int32), ptr(out, C.c_double)
+    launches = C.c_int(0)
+    check(load().dcs_contract_batch(len(items), arr, C.byref(launches)))
+    return outs, launches.value
 
 
 def ptr(a, ctype):

@@ -236,7 +236,8 @@ class ConditionalBlockGPU:
             self._plan.close()
             self._plan = None
         if getattr(self, 'ctx', None) is not None:
-            self.ctx.close()
+            if getattr(self, '_own_ctx', True):
+                self.ctx.close()
             self.ctx = None
 
 
@@ -247,7 +248,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
               fuse_max_dim=None, resident_bits=12, tile_log2=None, blocks=1, exact_rounding=False,
-              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=3, graphs=False):
+              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=3, graphs=False, context=None):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:85-105).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
@@ -276,6 +277,8 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    arithmetic as 1, so the same blocks bit for bit; 4 as 3 with the last register pass
                    storing to HBM from registers and the scale folded into the group's first step
         graphs     replay each slice's launch sequence from a captured HIP graph (one per lane count)
+        context    a _native.Context to share (engines contracted together by block_batch must share
+                   one); the engine then leaves it open on close().  Default: a context of its own
         The legacy fp64_kernel_path argument is accepted and ignored, as in the reference."""
         dt = np.dtype(dtype)
         if dt not in (np.dtype(np.complex64), np.dtype(np.complex128)):
@@ -294,7 +297,8 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
         fuse_max_dim = 8 if fuse_max_dim is None else fuse_max_dim
         tile_log2 = 12 if tile_log2 is None else tile_log2
         self.lanes = int(lanes)
-        self.ctx = _native.Context(device)
+        self._own_ctx = context is None
+        self.ctx = _native.Context(device) if context is None else context
         storage = _native.STORAGE_C128 if self.use_fp64 else self._c64_storage
         self._plan = _native.Plan(self.ctx, bp.step_table(), bp.inplace_maxslots, storage, self.lanes, bp.k,
                                   bp.terminal_mask(), bp.syndrome_map(), bp.phase(), bp.open_unitaries(),
@@ -357,3 +361,72 @@ class HalfConditionalBlockGPU(InplaceConditionalBlockGPU):
                      'coherent_accumulation': 'complex128',
                      'approximation_scope': 'Whole-vector quality requires independent FP64 validation'})
         return vector, meta
+
+
+def block_batch(requests, w=None):
+    """Blocks of prefixes of several circuits at once, the circuits' contractions sharing launches.
+
+    requests  [(engine, x_bits, block_index)]: InplaceConditionalBlockGPU engines of different plans
+              (different frontier widths allowed), allocated with schedule='fused' on one shared context
+              (alloc(context=...)) with the default tile shapes; an engine may appear more than once,
+              with distinct block indices.
+    Each round contracts up to `lanes` pending slices of every engine in one dcs_contract_batch call, whose
+    launches each run the same schedule segment of all engines together; every slice is then scattered
+    into its request's block in the order block() uses, and each block gets its tail.  Every block equals
+    engine.block(x_bits, block_index=b) alone, bit for bit.  Returns ([DeviceBlock], meta)."""
+    if not requests:
+        return [], {'seconds': 0.0, 'launches': 0, 'rounds': 0}
+    engines = []
+    for e, _, _ in requests:
+        if e._plan is None:
+            raise RuntimeError('call alloc() first')
+        if e.mode != _native.MODE_FUSED:
+            raise ValueError('block_batch runs engines allocated with schedule="fused"')
+        if all(e is not f for f in engines):
+            engines.append(e)
+    ctx = engines[0].ctx
+    if any(e.ctx is not ctx for e in engines):
+        raise ValueError('engines contracted together must share one context: alloc(context=...)')
+    if len(engines) > _native.BATCH_MAX:
+        raise ValueError(f'at most {_native.BATCH_MAX} engines per batch')
+    seen = set()
+    for e, _, b in requests:
+        if (id(e), b) in seen:
+            raise ValueError(f'block index {b} requested twice for one engine')
+        if not 0 <= b < e._plan.blocks:
+            raise ValueError(f'block index {b} outside 0..{e._plan.blocks - 1}')
+        seen.add((id(e), b))
+    start = time.perf_counter()
+    pending = {id(e): [] for e in engines}
+    counts = []
+    for e, x, b in requests:
+        jobs, count = e._slice_jobs(x, w, 0, 1)
+        pending[id(e)].extend((b,) + j for j in jobs)
+        counts.append(count)
+        e._plan.block_zero(b)
+    launches = rounds = 0
+    while any(pending.values()):
+        taken, items = [], []
+        for e in engines:
+            jobs = pending[id(e)][:e.lanes]
+            if not jobs:
+                continue
+            pending[id(e)] = pending[id(e)][e.lanes:]
+            taken.append((e, jobs))
+            items.append((e._plan, np.arange(len(jobs), dtype=np.int32), np.stack([j[2] for j in jobs]),
+                          np.stack([j[3] for j in jobs]), np.stack([j[4] for j in jobs])))
+        m2s, n = _native.contract_batch(items)
+        launches += n
+        rounds += 1
+        for (e, jobs), m2 in zip(taken, m2s):
+            for lane, (b, coef, _, _, _, offset, freeflip) in enumerate(jobs):
+                ls, alive = logscale_from_maxima(m2[lane])
+                if not alive:
+                    continue
+                e._plan.block_accumulate(b, lane, (offset >> e.stop) ^ freeflip, coef * e.bp.slice_scale(ls))
+    for e, _, b in requests:
+        e._plan.block_transform(b)
+    ctx.synchronize()
+    return [DeviceBlock(e, b) for e, _, b in requests], {
+        'seconds': time.perf_counter() - start, 'slices_evaluated': counts, 'launches': launches,
+        'rounds': rounds}

@@ -282,15 +282,14 @@ __device__ __forceinline__ double block_max(double lm, double *red) {
     return m;
 }
 
+// A run of resident steps s0..s1-1 on lane `li`: the frontier of the lane is loaded into LDS once, every
+// step runs there with the exact per-step scale chain, and the live slots are written back.
 template <class S, int RB>
-__global__ void __launch_bounds__(kThreads) k_resident(S *__restrict__ base, u64 lane_stride,
-                                                       const int *__restrict__ lane_ids,
-                                                       const StepDev *__restrict__ steps, int s0, int s1,
-                                                       const Dyn *__restrict__ dyn, int nsteps,
-                                                       u64 *__restrict__ dmax, int nunits, int unit0) {
-    __shared__ S tile[1 << RB];
-    __shared__ double red[kThreads / 64];
-    const int li = blockIdx.x;
+__device__ __forceinline__ void resident_run(S *__restrict__ tile, double *__restrict__ red, S *__restrict__ base,
+                                             u64 lane_stride, const int *__restrict__ lane_ids,
+                                             const StepDev *__restrict__ steps, int s0, int s1,
+                                             const Dyn *__restrict__ dyn, int nsteps, u64 *__restrict__ dmax,
+                                             int nunits, int unit0, int li) {
     S *__restrict__ E = base + (u64)lane_ids[li] * lane_stride;
     const u64 live_in = steps[s0].in_mask;
     for (u64 e = threadIdx.x; e < (1ull << RB); e += blockDim.x)
@@ -311,22 +310,30 @@ __global__ void __launch_bounds__(kThreads) k_resident(S *__restrict__ base, u64
         if (!(e & ~live_out)) E[e] = tile[e];
 }
 
+template <class S, int RB>
+__global__ void __launch_bounds__(kThreads) k_resident(S *__restrict__ base, u64 lane_stride,
+                                                       const int *__restrict__ lane_ids,
+                                                       const StepDev *__restrict__ steps, int s0, int s1,
+                                                       const Dyn *__restrict__ dyn, int nsteps,
+                                                       u64 *__restrict__ dmax, int nunits, int unit0) {
+    __shared__ S tile[1 << RB];
+    __shared__ double red[kThreads / 64];
+    resident_run<S, RB>(tile, red, base, lane_stride, lane_ids, steps, s0, s1, dyn, nsteps, dmax, nunits, unit0,
+                        blockIdx.x);
+}
+
 // ------------------------------------------------------------------------------------------------
 // k_coset: fused group.  Tile layout: element (t, l) at LDS index t*W + l, t in [0, 2^d) the
 // coordinate in the RREF basis of V (address = c0(l) ^ addrv[t]), l the lane coordinate.
+// Tiles slot, slot + nslots, ... of group gi on lane `li` (the persistent loop of one workgroup).
 template <class S, int TL, bool EXACT, int JMAX>
-__global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 lane_stride,
-                                                    const int *__restrict__ lane_ids,
-                                                    const GroupDev *__restrict__ groups, int gi,
-                                                    const PassDev *__restrict__ passes,
-                                                    const PStepDev *__restrict__ psteps,
-                                                    const u64 *__restrict__ addrv_all,
-                                                    const Dyn *__restrict__ dyn, int nsteps,
-                                                    u64 *__restrict__ dmax, int nunits, int unit) {
-    __shared__ double2 tile[1 << TL];
-    __shared__ u64 addrv[1 << kMaxDim];
+__device__ __forceinline__ void coset_run(double2 *__restrict__ tile, u64 *__restrict__ addrv, S *__restrict__ base,
+                                          u64 lane_stride, const int *__restrict__ lane_ids,
+                                          const GroupDev *__restrict__ groups, int gi,
+                                          const PassDev *__restrict__ passes, const PStepDev *__restrict__ psteps,
+                                          const u64 *__restrict__ addrv_all, const Dyn *__restrict__ dyn, int nsteps,
+                                          u64 *__restrict__ dmax, int nunits, int unit, int li, u64 slot, u64 nslots) {
     const GroupDev &g = groups[gi];
-    const int li = blockIdx.y;
     S *__restrict__ E = base + (u64)lane_ids[li] * lane_stride;
     const Dyn *dyn_lane = dyn + (u64)li * nsteps;
     const int d = g.d, wb = g.wbits;
@@ -356,7 +363,7 @@ __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 la
             R[k] = v;
         }
     };
-    u64 ti = blockIdx.x;
+    u64 ti = slot;
     u64 tb = 0;
     if (ti < g.ntiles) {
         tb = deposit(ti, g.high);
@@ -372,7 +379,7 @@ __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 la
             }
         }
         __syncthreads();
-        const u64 tn = ti + gridDim.x;
+        const u64 tn = ti + nslots;
         u64 tb_next = 0;
         if (tn < g.ntiles) {
             tb_next = deposit(tn, g.high);
@@ -400,6 +407,71 @@ __global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 la
         tb = tb_next;
     }
     commit_max(lm, &dmax[(u64)li * nunits + unit]);
+}
+
+template <class S, int TL, bool EXACT, int JMAX>
+__global__ void __launch_bounds__(kThreads) k_coset(S *__restrict__ base, u64 lane_stride,
+                                                    const int *__restrict__ lane_ids,
+                                                    const GroupDev *__restrict__ groups, int gi,
+                                                    const PassDev *__restrict__ passes,
+                                                    const PStepDev *__restrict__ psteps,
+                                                    const u64 *__restrict__ addrv_all,
+                                                    const Dyn *__restrict__ dyn, int nsteps,
+                                                    u64 *__restrict__ dmax, int nunits, int unit) {
+    __shared__ double2 tile[1 << TL];
+    __shared__ u64 addrv[1 << kMaxDim];
+    coset_run<S, TL, EXACT, JMAX>(tile, addrv, base, lane_stride, lane_ids, groups, gi, passes, psteps, addrv_all, dyn,
+                                  nsteps, dmax, nunits, unit, blockIdx.y, blockIdx.x, gridDim.x);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Launches shared by several plans (dcs_contract_batch).  One entry per plan: its frontier, lanes,
+// tables, parameters and maxima, and the segment it runs in this launch.  Workgroups
+// [begin, begin + nlanes * blocks) of the launch belong to the entry; workgroup begin + r serves lane
+// r / blocks at tile slot r % blocks.  The plans differ in width, group shapes and tile counts; each
+// entry's segment runs through the same device code as the single-plan kernels.
+struct BatchEntry {
+    void *base;
+    u64 lane_stride;
+    const int *lanes;
+    const StepDev *steps;
+    const GroupDev *groups;
+    const PassDev *passes;
+    const PStepDev *psteps;
+    const u64 *addrv;
+    const Dyn *dyn;
+    u64 *dmax;
+    int32_t nsteps, nunits, unit;
+    int32_t first, last;  // coset: group index (first); resident run: steps first .. last-1
+    int32_t nlanes;
+    uint32_t blocks, begin;
+};
+
+// Entry of this workgroup: the last one whose first workgroup is not after it (entries are few).
+__device__ __forceinline__ int batch_entry(const BatchEntry *__restrict__ e, int n) {
+    int i = 0;
+    while (i + 1 < n && blockIdx.x >= e[i + 1].begin) ++i;
+    return i;
+}
+
+template <class S, int RB>
+__global__ void __launch_bounds__(kThreads) k_resident_batch(const BatchEntry *__restrict__ ents, int n) {
+    __shared__ S tile[1 << RB];
+    __shared__ double red[kThreads / 64];
+    const BatchEntry &b = ents[batch_entry(ents, n)];
+    resident_run<S, RB>(tile, red, (S *)b.base, b.lane_stride, b.lanes, b.steps, b.first, b.last, b.dyn, b.nsteps,
+                        b.dmax, b.nunits, b.unit, (int)(blockIdx.x - b.begin));
+}
+
+template <class S, int TL, bool EXACT, int JMAX>
+__global__ void __launch_bounds__(kThreads) k_coset_batch(const BatchEntry *__restrict__ ents, int n) {
+    __shared__ double2 tile[1 << TL];
+    __shared__ u64 addrv[1 << kMaxDim];
+    const BatchEntry &b = ents[batch_entry(ents, n)];
+    const uint32_t r = blockIdx.x - b.begin;
+    coset_run<S, TL, EXACT, JMAX>(tile, addrv, (S *)b.base, b.lane_stride, b.lanes, b.groups, b.first, b.passes,
+                                  b.psteps, b.addrv, b.dyn, b.nsteps, b.dmax, b.nunits, b.unit, (int)(r / b.blocks),
+                                  r % b.blocks, b.blocks);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -795,6 +867,9 @@ struct dcs_context {
     int cus;
     void *scratch;      // this device's address of the static scratch (dcs_draw_noalloc)
     DrawOut *h_draw;    // pinned result of dcs_draw_noalloc
+    // entries of dcs_contract_batch's launches: pinned staging and device copy, grown on demand
+    BatchEntry *h_batch = nullptr, *d_batch = nullptr;
+    size_t batch_cap = 0;
 };
 
 namespace {
@@ -1596,6 +1671,144 @@ int run_contract(dcs_plan *p, int mode, int nl, std::vector<u64> &ntiles_of_grou
 
 int enqueue_contract(dcs_plan *p, int mode, int nl, int nunits_out);
 
+// Lane ids of a contraction: within the plan's lanes and distinct.
+int check_lanes(const dcs_plan *p, int nl, const int32_t *lanes) {
+    if (nl < 1 || nl > p->lanes) return fail(DCS_ERR_INVALID, "nlanes %d outside 1..%d", nl, p->lanes);
+    for (int i = 0; i < nl; ++i) {
+        if (lanes[i] < 0 || lanes[i] >= p->lanes) return fail(DCS_ERR_INVALID, "lane %d out of range", lanes[i]);
+        for (int j = 0; j < i; ++j)
+            if (lanes[j] == lanes[i]) return fail(DCS_ERR_INVALID, "lane %d repeated", lanes[i]);
+    }
+    return DCS_OK;
+}
+
+// Per-lane step parameters and lane ids into the plan's pinned staging buffers.
+void stage_parameters(dcs_plan *p, int nl, const int32_t *lanes, const double *coeffs, const int32_t *ov,
+                      const int32_t *y) {
+    const int n = p->nsteps;
+    for (int li = 0; li < nl; ++li)
+        for (int s = 0; s < n; ++s) {
+            Dyn &d = p->h_dyn[(size_t)li * n + s];
+            const double *c = coeffs + ((size_t)li * n + s) * 4;
+            d.f0r = c[0];
+            d.f0i = c[1];
+            d.f1r = c[2];
+            d.f1i = c[3];
+            d.ov = ov[(size_t)li * n + s] & 1;
+            d.y = y[(size_t)li * n + s] & 1;
+            d.pad[0] = d.pad[1] = 0;
+        }
+    for (int i = 0; i < nl; ++i) p->h_lanes[i] = lanes[i];
+}
+
+// One launch of dcs_contract_batch: entries [first, first + count) of the round, all of one segment type.
+struct BatchLaunch {
+    int type, first, count;
+    uint32_t workgroups;
+};
+
+// The fused schedules of several plans in lockstep: round r runs segment r of every plan that has one,
+// the resident runs of the round in one k_resident_batch launch and the coset groups in one
+// k_coset_batch launch.  A plan's segments stay in order (one stream), and the plans of a round touch
+// disjoint memory, so the two launches of a round may run in either order.
+template <class S>
+int run_batch(dcs_context *ctx, int n, const dcs_batch_item *it, int *launches) {
+    hipStream_t st = ctx->stream;
+    for (int i = 0; i < n; ++i) {
+        dcs_plan *p = it[i].plan;
+        const int nl = it[i].nlanes;
+        HIPCHK(hipMemcpyAsync(p->d_dyn, p->h_dyn, sizeof(Dyn) * (size_t)nl * p->nsteps, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(p->d_lanes, p->h_lanes, sizeof(int) * nl, hipMemcpyHostToDevice, st));
+        const u64 cnt = (u64)nl * p->units_fused;
+        k_fill_u64<<<grid_for(cnt, ctx->cus, 4), kThreads, 0, st>>>(p->d_dmax, cnt, 0ull);
+        HIPCHK(hipGetLastError());
+        k_init_lanes<S><<<1, 64, 0, st>>>((S *)p->frontier, p->lane_stride, p->d_lanes, nl);
+        HIPCHK(hipGetLastError());
+    }
+    std::vector<BatchEntry> ents;
+    std::vector<BatchLaunch> plan_launches;
+    for (size_t r = 0;; ++r) {
+        bool any = false;
+        for (int type = 1; type <= 2; ++type) {
+            BatchLaunch L{type, (int)ents.size(), 0, 0};
+            for (int i = 0; i < n; ++i) {
+                const dcs_plan *p = it[i].plan;
+                if (r >= p->sched_fused.size() || p->sched_fused[r].type != type) continue;
+                const Seg &sg = p->sched_fused[r];
+                BatchEntry e;
+                memset(&e, 0, sizeof(e));
+                e.base = p->frontier;
+                e.lane_stride = p->lane_stride;
+                e.lanes = p->d_lanes;
+                e.steps = p->d_steps;
+                e.groups = p->d_groups;
+                e.passes = p->d_passes;
+                e.psteps = p->d_psteps;
+                e.addrv = p->d_addrv;
+                e.dyn = p->d_dyn;
+                e.dmax = p->d_dmax;
+                e.nsteps = p->nsteps;
+                e.nunits = p->units_fused;
+                e.unit = sg.unit;
+                e.nlanes = it[i].nlanes;
+                if (type == 1) {
+                    e.first = sg.first;
+                    e.last = sg.first + sg.count;
+                    e.blocks = 1;
+                } else {
+                    if (sg.dim > kMaxDim)
+                        return fail(DCS_ERR_INVALID, "item %d: coset dimension %d exceeds %d (fuse_max_dim)", i, sg.dim,
+                                    kMaxDim);
+                    e.first = sg.group;
+                    e.blocks = (uint32_t)std::max<u64>(1, std::min<u64>(p->group_ntiles[sg.group], (u64)ctx->cus * 8));
+                }
+                e.begin = L.workgroups;
+                const u64 total = (u64)L.workgroups + (u64)e.blocks * (u64)e.nlanes;
+                if (total > 0x7fffffffull) return fail(DCS_ERR_INVALID, "batched launch exceeds the grid limit");
+                L.workgroups = (uint32_t)total;
+                ents.push_back(e);
+                ++L.count;
+                any = true;
+            }
+            if (L.count) plan_launches.push_back(L);
+        }
+        if (!any) break;
+    }
+    if (ents.size() > ctx->batch_cap) {
+        if (ctx->h_batch) (void)hipHostFree(ctx->h_batch);
+        if (ctx->d_batch) (void)hipFree(ctx->d_batch);
+        ctx->h_batch = ctx->d_batch = nullptr;
+        ctx->batch_cap = 0;
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipHostMalloc((void **)&ctx->h_batch, sizeof(BatchEntry) * ents.size(), hipHostMallocDefault));
+        HIPCHK(hipMalloc((void **)&ctx->d_batch, sizeof(BatchEntry) * ents.size()));
+        ctx->batch_cap = ents.size();
+    }
+    // the staging buffer may still feed an earlier batch's copy
+    HIPCHK(hipStreamSynchronize(st));
+    if (!ents.empty()) {
+        memcpy(ctx->h_batch, ents.data(), sizeof(BatchEntry) * ents.size());
+        HIPCHK(hipMemcpyAsync(ctx->d_batch, ctx->h_batch, sizeof(BatchEntry) * ents.size(), hipMemcpyHostToDevice, st));
+    }
+    const dcs_plan *p0 = it[0].plan;
+    for (const BatchLaunch &L : plan_launches) {
+        const BatchEntry *e = ctx->d_batch + L.first;
+        // the shapes dcs_contract_batch accepts: resident_bits 12, tile_log2 12 or 11, pass_max_dim 3, no
+        // per-step rounding (one instantiation each, to keep the library small)
+        if (L.type == 1) k_resident_batch<S, 12><<<L.workgroups, kThreads, 0, st>>>(e, L.count);
+        else if (p0->tile_log2 == 12) k_coset_batch<S, 12, false, 3><<<L.workgroups, kThreads, 0, st>>>(e, L.count);
+        else k_coset_batch<S, 11, false, 3><<<L.workgroups, kThreads, 0, st>>>(e, L.count);
+        HIPCHK(hipGetLastError());
+    }
+    for (int i = 0; i < n; ++i) {
+        dcs_plan *p = it[i].plan;
+        HIPCHK(hipMemcpyAsync(p->h_dmax, p->d_dmax, sizeof(u64) * (size_t)it[i].nlanes * p->units_fused,
+                              hipMemcpyDeviceToHost, st));
+    }
+    if (launches) *launches = (int)plan_launches.size();
+    return DCS_OK;
+}
+
 }  // namespace
 
 // ================================================================================================
@@ -1658,6 +1871,8 @@ void dcs_context_destroy(dcs_context *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipHostFree(ctx->h_draw);
+    if (ctx->h_batch) (void)hipHostFree(ctx->h_batch);
+    if (ctx->d_batch) (void)hipFree(ctx->d_batch);
     delete ctx;
 }
 
@@ -2076,31 +2291,16 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
     if (mode < 0 || mode > 3) return fail(DCS_ERR_INVALID, "mode %d", mode);
     if (p->executor == 1 && mode != DCS_MODE_STEPS && mode != DCS_MODE_STEPS_HOST)
         return fail(DCS_ERR_INVALID, "a gather plan runs DCS_MODE_STEPS or DCS_MODE_STEPS_HOST");
-    if (nl < 1 || nl > p->lanes) return fail(DCS_ERR_INVALID, "nlanes %d outside 1..%d", nl, p->lanes);
-    for (int i = 0; i < nl; ++i) {
-        if (lanes[i] < 0 || lanes[i] >= p->lanes) return fail(DCS_ERR_INVALID, "lane %d out of range", lanes[i]);
-        for (int j = 0; j < i; ++j)
-            if (lanes[j] == lanes[i]) return fail(DCS_ERR_INVALID, "lane %d repeated", lanes[i]);
+    {
+        const int rc = check_lanes(p, nl, lanes);
+        if (rc) return rc;
     }
     HIPCHK(hipSetDevice(p->ctx->device));
     if (mode == DCS_MODE_COMPILED && !p->compiled) {
         const int rc = compile_groups(p, nullptr);
         if (rc) return rc;
     }
-    const int n = p->nsteps;
-    for (int li = 0; li < nl; ++li)
-        for (int s = 0; s < n; ++s) {
-            Dyn &d = p->h_dyn[(size_t)li * n + s];
-            const double *c = coeffs + ((size_t)li * n + s) * 4;
-            d.f0r = c[0];
-            d.f0i = c[1];
-            d.f1r = c[2];
-            d.f1i = c[3];
-            d.ov = ov[(size_t)li * n + s] & 1;
-            d.y = y[(size_t)li * n + s] & 1;
-            d.pad[0] = d.pad[1] = 0;
-        }
-    for (int i = 0; i < nl; ++i) p->h_lanes[i] = lanes[i];
+    stage_parameters(p, nl, lanes, coeffs, ov, y);
     hipStream_t st = p->ctx->stream;
     const int nunits_out = (mode >= DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
     // Graph replay: the whole sequence below (parameter upload from the pinned staging buffers, maxima
@@ -2139,6 +2339,53 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
     HIPCHK(hipStreamSynchronize(st));
     memcpy(unit_m2, p->h_dmax, sizeof(double) * (size_t)nl * nunits_out);
     return ev_resolve(p);
+}
+
+int dcs_contract_batch(int n, const dcs_batch_item *items, int *launches) {
+    if (!items || n < 1 || n > DCS_BATCH_MAX) return fail(DCS_ERR_INVALID, "batch of %d items outside 1..%d", n, DCS_BATCH_MAX);
+    const dcs_plan *p0 = items[0].plan;
+    if (!p0) return fail(DCS_ERR_INVALID, "item 0: null plan");
+    for (int i = 0; i < n; ++i) {
+        const dcs_batch_item &b = items[i];
+        const dcs_plan *p = b.plan;
+        if (!p || !b.lanes || !b.coeffs || !b.ov || !b.y || !b.unit_m2)
+            return fail(DCS_ERR_INVALID, "item %d: null argument", i);
+        if (p->executor != 0) return fail(DCS_ERR_INVALID, "item %d: a gather plan cannot be batched", i);
+        if (p->ctx != p0->ctx) return fail(DCS_ERR_INVALID, "item %d: batched plans must share one context", i);
+        if (p->storage != p0->storage || p->tile_log2 != p0->tile_log2 || p->pass_max_dim != p0->pass_max_dim ||
+            p->exact_rounding != p0->exact_rounding || p->resident_bits != p0->resident_bits)
+            return fail(DCS_ERR_INVALID,
+                        "item %d: batched plans must share storage, tile_log2, pass_max_dim, exact_rounding and "
+                        "resident_bits", i);
+        for (int j = 0; j < i; ++j)
+            if (items[j].plan == p) return fail(DCS_ERR_INVALID, "item %d: plan repeated (give it several lanes)", i);
+        const int rc = check_lanes(p, b.nlanes, b.lanes);
+        if (rc) return rc;
+        for (const Seg &sg : p->sched_fused)
+            if (sg.type == 2 && sg.dim > kMaxDim)
+                return fail(DCS_ERR_INVALID, "item %d: coset dimension %d exceeds %d: use a smaller fuse_max_dim", i,
+                            sg.dim, kMaxDim);
+    }
+    if ((p0->tile_log2 != 11 && p0->tile_log2 != 12) || p0->pass_max_dim != 3 || p0->exact_rounding ||
+        p0->resident_bits != 12 || p0->storage == DCS_STORAGE_F16X2)
+        return fail(DCS_ERR_INVALID,
+                    "batched plans: complex64 or complex128 storage, tile_log2 11 or 12, pass_max_dim 3, "
+                    "resident_bits 12, no exact_rounding");
+    dcs_context *ctx = p0->ctx;
+    HIPCHK(hipSetDevice(ctx->device));
+    for (int i = 0; i < n; ++i)
+        stage_parameters(items[i].plan, items[i].nlanes, items[i].lanes, items[i].coeffs, items[i].ov, items[i].y);
+    int rc;
+    if (p0->storage == DCS_STORAGE_C64) rc = run_batch<float2>(ctx, n, items, launches);
+    else rc = run_batch<double2>(ctx, n, items, launches);
+    if (rc) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return rc;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i)
+        memcpy(items[i].unit_m2, items[i].plan->h_dmax, sizeof(double) * (size_t)items[i].nlanes * items[i].plan->units_fused);
+    return DCS_OK;
 }
 
 int dcs_plan_use_graphs(dcs_plan *p, int enable) {

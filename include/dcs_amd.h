@@ -26,7 +26,8 @@
  *   dcs_contract on a gather  ConditionalBlockGPU._prefix (one launch   sampling/conditional_block_gpu.py:92-126
  *   plan                      of scan_step_scaled per step)            dcs_rankwidth/gpu_kernel.py:58-101
  * dcs_plan_kernel_source has no counterpart in the reference: it returns the generated group kernels
- * of a plan without a device, for inspection and for emulating them on the CPU.
+ * of a plan without a device, for inspection and for emulating them on the CPU.  Neither has
+ * dcs_contract_batch (the lanes of several plans, circuits of different widths, in shared launches).
  */
 #ifndef DCS_AMD_H
 #define DCS_AMD_H
@@ -207,6 +208,32 @@ int dcs_plan_schedule(dcs_plan *plan, int mode, int cap, int32_t *seg_kind, int3
  * Returns after the device finished. */
 int dcs_contract(dcs_plan *plan, int mode, int nlanes, const int32_t *lanes, const double *coeffs,
                  const int32_t *ov, const int32_t *y, double *unit_m2);
+
+/* One plan's part of dcs_contract_batch: its lanes and their step parameters, laid out as in
+ * dcs_contract, and its unit maxima out ([nlanes][nunits of DCS_MODE_FUSED]). */
+typedef struct dcs_batch_item {
+    dcs_plan *plan;
+    int nlanes;
+    const int32_t *lanes;
+    const double *coeffs;
+    const int32_t *ov;
+    const int32_t *y;
+    double *unit_m2;
+} dcs_batch_item;
+
+#define DCS_BATCH_MAX 16
+
+/* Contract the lanes of up to DCS_BATCH_MAX different plans (circuits of different frontier widths)
+ * together, in the fused schedule on the generic kernels: round r runs segment r of every plan's
+ * schedule, all resident runs of the round in one launch and all coset groups of the round in one
+ * launch, so plans too narrow to fill the GPU alone share its compute units.  Each lane's result is
+ * bit-identical to dcs_contract(plan, DCS_MODE_FUSED, ...) of the same parameters.  The plans must be
+ * distinct, share one context and one storage (complex64 or complex128) and tile_log2 (11 or 12), and
+ * have pass_max_dim 3, resident_bits 12 and no exact_rounding (the defaults).  *launches (may be NULL)
+ * receives the number of contraction launches.  Returns
+ * after the device finished.  No counterpart in the reference, which contracts one prefix per call
+ * (kernel_opt/inplace_gpu.py:107-124); it serves a caller holding requests for several small circuits. */
+int dcs_contract_batch(int nitems, const dcs_batch_item *items, int *launches);
 
 /* 1: dcs_contract captures its launch sequence (parameter upload, maxima clear, every kernel of the
  * schedule, maxima read-back) into a HIP graph once per (mode, nlanes) and replays it on later calls;

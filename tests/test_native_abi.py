@@ -33,7 +33,8 @@ def test_structure_layouts(tmp_path):
     header's structure (gcc on a probe that includes include/dcs_amd.h)."""
     import subprocess
     pairs = {'dcs_draw_result': _native.DrawResult, 'dcs_device_info': _native.DeviceInfo,
-             'dcs_plan_desc': _native.PlanDesc, 'dcs_gather_desc': _native.GatherDesc}
+             'dcs_plan_desc': _native.PlanDesc, 'dcs_gather_desc': _native.GatherDesc,
+             'dcs_batch_item': _native.BatchItem}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', 'int main(void) {']
     for cname, cls in pairs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
