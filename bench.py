@@ -252,8 +252,6 @@ def config_lines(repeats=3):
                           'gpu_over_cpu': cpu_s / gpu_s, 'max_rel_amp_diff_gpu_vs_cpu_c64': err}), flush=True)
 
 
-BATCH_MIX = [(16, 12, 20, 0, 8), (20, 16, 40, 1, 10), (24, 24, 80, 3, 8), (28, 32, 96, 1, 10),
-             (32, 30, 100, 0, 12), (24, 20, 60, 2, 10), (28, 24, 80, 2, 10), (32, 28, 90, 1, 12)]
 
 
 def batch_line(prefixes=4, repeats=5):
@@ -265,10 +263,11 @@ def batch_line(prefixes=4, repeats=5):
     from dcs_amd.block_plan import BlockPlan
     from dcs_amd.conditional_block import InplaceConditionalBlockGPU, block_batch
     from dcs_amd.sampling_protocol import make_protocol
+    from dcs_amd.serving import NARROW_MIX
     from dcs_amd.stabilizer import synthetic_instance
     from dcs_amd.width_planner import best_rotation
     cases = []
-    for n, d, t, seed, k in BATCH_MIX:
+    for n, d, t, seed, k in NARROW_MIX:
         inst = synthetic_instance(n, d, t, seed)
         order, opn, _ = best_rotation(inst['graph'], inst['unitaries'], n, inst['natural_order'], k, stride=2)
         args = (inst['graph'], inst['unitaries'], n, order, [], opn)
@@ -317,7 +316,7 @@ def batch_line(prefixes=4, repeats=5):
         e.close()
     print(json.dumps({'config': 'batch of small circuits of uneven width',
                       'circuits': [f'{n}q/d{d}/{t}T seed {s}, width {w}, 2^{k} block'
-                                   for (n, d, t, s, k), (_, _, w) in zip(BATCH_MIX, cases)],
+                                   for (n, d, t, s, k), (_, _, w) in zip(NARROW_MIX, cases)],
                       'requests': nreq, 'storage': 'complex64',
                       'block_batch': {'samples_per_s': nreq / t_batch, 'ms_per_round': 1e3 * t_batch,
                                       'launches': meta.get('launches'), 'rounds': meta.get('rounds')},

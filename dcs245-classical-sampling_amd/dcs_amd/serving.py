@@ -18,6 +18,11 @@ process keeps several circuits resident on the GPU and serves a stream of reques
     the same time (the library releases the interpreter lock during device calls) and a short request
     no longer waits behind another circuit's batch; arrivals are released in real time.
 
+  * Server(narrow_width=w) serves circuits whose frontier width is at most w together: they are
+    allocated on one shared context in the fused schedule, and one batch takes the waiting requests of
+    all of them through block_batch (their schedules' segments in shared launches), since a narrow
+    circuit alone leaves most of the GPU idle.  Off by default.
+
 `poisson_workload` generates seeded arrivals (exponential gaps) over a weighted circuit mix, with
 prefixes and uniforms drawn from PCG64 in the order of the reference protocol (sampling_protocol.py:21-33).
 """
@@ -65,10 +70,12 @@ def footprint_bytes(bp, dtype, lanes, blocks):
 
 
 class Circuit:
-    """A resident circuit: its engine holds frontiers, plan tables and block accumulators on the device."""
+    """A resident circuit: its engine holds frontiers, plan tables and block accumulators on the device.
+    narrow: served with the other narrow circuits in shared launches (Server(narrow_width=...))."""
 
-    def __init__(self, name, engine, lanes, max_batch, nbytes):
+    def __init__(self, name, engine, lanes, max_batch, nbytes, narrow=False):
         self.name, self.engine, self.lanes, self.max_batch, self.nbytes = name, engine, lanes, max_batch, nbytes
+        self.narrow = narrow
         self.served = 0
         self.last_used = time.perf_counter()
 
@@ -96,7 +103,11 @@ class Server:
     in flight) is evicted, least recently used first, when a waiting circuit needs its memory.
     register() adds a circuit and admits it at once."""
 
-    def __init__(self, device=0, hbm_budget=None, reserve=8 << 30, engine_factory=None):
+    def __init__(self, device=0, hbm_budget=None, reserve=8 << 30, engine_factory=None, narrow_width=0,
+                 batch_fn=None, context_factory=None):
+        """narrow_width: circuits whose in-place frontier width is at most this are served together in
+        shared launches (block_batch; 0 = never).  batch_fn and context_factory replace block_batch and
+        the shared device context (stand-ins for host tests)."""
         self.device = device
         if hbm_budget is None:
             hbm_budget = _native.device_info(device)['free_bytes'] - reserve
@@ -106,6 +117,19 @@ class Server:
         self.circuits = collections.OrderedDict()
         self.evictions = 0
         self._factory = engine_factory or InplaceConditionalBlockGPU
+        self.narrow_width = int(narrow_width)
+        self.narrow_batches = 0
+        self._batch_fn = batch_fn
+        self._context_factory = context_factory or _native.Context
+        self._shared_ctx = None
+
+    def _narrow(self, spec):
+        return self.narrow_width > 0 and spec.engine.bp.inplace_maxslots <= self.narrow_width
+
+    def _shared_context(self):
+        if self._shared_ctx is None:
+            self._shared_ctx = self._context_factory(self.device)
+        return self._shared_ctx
 
     # ---- admission ---------------------------------------------------------------------------
     def add(self, name, graph, unitaries, n, order, parity_operations, open_vertices, dtype=np.complex64,
@@ -174,7 +198,12 @@ class Server:
             self._evict(v)
         lanes = self._fitting_lanes(spec, self.budget - self.used)
         alloc_kw = dict(spec.alloc_kw)
-        if spec.autotune:
+        narrow = self._narrow(spec)
+        if narrow:
+            # shared launches run the fused schedule's generic kernels with their default tile shapes
+            alloc_kw = {k: v for k, v in alloc_kw.items() if k == 'group_policy'}
+            alloc_kw.update(schedule='fused', context=self._shared_context())
+        elif spec.autotune:
             # candidates are timed with the lane and block counts this circuit is admitted with
             from .autotune import autotune as tune
             shared = {k: v for k, v in alloc_kw.items() if k in ('lds_storage', 'pass_max_dim', 'resident_bits')}
@@ -186,7 +215,7 @@ class Server:
         nbytes = spec.engine.device_bytes() or footprint_bytes(spec.engine.bp, spec.dtype, lanes, spec.max_batch)
         self.used += nbytes
         spec.admissions += 1
-        c = Circuit(name, spec.engine, lanes, spec.max_batch, nbytes)
+        c = Circuit(name, spec.engine, lanes, spec.max_batch, nbytes, narrow=narrow)
         self.circuits[name] = c
         return c
 
@@ -204,6 +233,9 @@ class Server:
     def close(self):
         for name in list(self.catalog):
             self.unregister(name)
+        if self._shared_ctx is not None:
+            self._shared_ctx.close()
+            self._shared_ctx = None
 
     # ---- the loop ------------------------------------------------------------------------------
     def serve(self, requests, realtime=True, keep_blocks=False, concurrent=False):
@@ -244,8 +276,17 @@ class Server:
                     break
             if c is None:
                 raise RuntimeError('no waiting circuit can be admitted although every resident one is idle')
-            batch = [q.popleft() for _ in range(min(c.max_batch, len(q)))]
-            self._run_batch(c, batch, clock, nbatch, keep_blocks)
+            if c.narrow:
+                # the waiting requests of every resident narrow circuit, in shared launches
+                groups = []
+                for name, nc in self.circuits.items():
+                    if nc.narrow and queues[name]:
+                        qn = queues[name]
+                        groups.append((nc, [qn.popleft() for _ in range(min(nc.max_batch, len(qn)))]))
+                self._run_narrow(groups, clock, nbatch, keep_blocks)
+            else:
+                batch = [q.popleft() for _ in range(min(c.max_batch, len(q)))]
+                self._run_batch(c, batch, clock, nbatch, keep_blocks)
             nbatch += 1
         return requests
 
@@ -265,12 +306,35 @@ class Server:
         c.served += len(batch)
         c.last_used = time.perf_counter()
 
+    def _run_narrow(self, groups, clock, nbatch, keep_blocks):
+        """One batch over several narrow circuits: their requests' blocks by block_batch (every circuit's
+        schedule segments in shared launches; each block bit-identical to its engine's block() alone),
+        then each request's device draw as in _run_batch."""
+        batch_fn = self._batch_fn
+        if batch_fn is None:
+            from .conditional_block import block_batch as batch_fn
+        start = clock()
+        work = [(c.engine, r.x_bits, i) for c, batch in groups for i, r in enumerate(batch)]
+        vecs, _ = batch_fn(work)
+        flat = [r for _, batch in groups for r in batch]
+        for r, v in zip(flat, vecs):
+            r.draw = v.draw(r.uniform)
+            if keep_blocks:
+                r.block = np.asarray(v)
+            r.index = r.draw['A_index']
+            r.start, r.done, r.batch = start, clock(), nbatch
+        now = time.perf_counter()
+        for c, batch in groups:
+            c.served += len(batch)
+            c.last_used = now
+        self.narrow_batches += 1
+
     def _serve_concurrent(self, requests, keep_blocks):
         todo = sorted(requests, key=lambda r: (r.arrival, r.rid))
         cv = threading.Condition()
         queues = {name: collections.deque() for name in self.catalog}
         running, stop, workers = set(), set(), {}
-        state = {'released': 0, 'batches': 0, 'error': None}
+        state = {'released': 0, 'batches': 0, 'error': None, 'narrow_alive': False, 'narrow_threads': 0}
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0
 
@@ -298,7 +362,50 @@ class Server:
                     running.discard(name)
                     cv.notify_all()
 
+        def narrow_ready():
+            return [n for n, c in self.circuits.items() if c.narrow and queues[n] and n not in stop and n not in running]
+
+        def narrow_worker():
+            # one thread serves every resident narrow circuit: each batch takes the waiting requests of all
+            # of them (shared launches); it leaves when nothing narrow is waiting and every request has been
+            # released, and the dispatcher starts a new one if a narrow circuit is admitted after that
+            while True:
+                with cv:
+                    cv.wait_for(lambda: narrow_ready() or state['error'] or state['released'] == len(todo))
+                    names = narrow_ready()
+                    if state['error'] or not names:
+                        state['narrow_alive'] = False
+                        return
+                    groups = []
+                    for n in names:
+                        c, q = self.circuits[n], queues[n]
+                        groups.append((c, [q.popleft() for _ in range(min(c.max_batch, len(q)))]))
+                    nb = state['batches']
+                    state['batches'] += 1
+                    running.update(names)
+                try:
+                    self._run_narrow(groups, clock, nb, keep_blocks)
+                except BaseException as exc:
+                    with cv:
+                        state['error'] = exc
+                        running.difference_update(names)
+                        state['narrow_alive'] = False
+                        cv.notify_all()
+                    return
+                with cv:
+                    running.difference_update(names)
+                    cv.notify_all()
+
         def start(name):
+            # called with the lock held, or before any worker runs
+            if self.circuits[name].narrow:
+                if not state['narrow_alive']:
+                    state['narrow_alive'] = True
+                    state['narrow_threads'] += 1
+                    t = threading.Thread(target=narrow_worker, daemon=True)
+                    workers[('narrow', state['narrow_threads'])] = t
+                    t.start()
+                return
             t = threading.Thread(target=worker, args=(name, self.circuits[name]), daemon=True)
             workers[name] = t
             t.start()
@@ -338,7 +445,9 @@ class Server:
                 # outside the lock: let the evicted circuits' workers leave, free them, admit the new one
                 name, victims = plan
                 for v in victims:
-                    workers.pop(v).join()
+                    t = workers.pop(v, None)    # narrow circuits have no worker of their own
+                    if t is not None:
+                        t.join()
                     self._evict(v)
                     stop.discard(v)
                 self._admit(name)
@@ -440,6 +549,46 @@ def c5_mix(srv, plan_cache=RING_PLANS, with_2_22=True, log=None):
     return mix, {'40q/d48/180T': w40, '52q/d60/250T': fit['width'], '52q_slices': fit['slices'], '64q': 34}
 
 
+# Seeded ring circuits of 16-32 qubits (qubits, depth, T count, seed, open qubits): frontier widths 7-19
+# with the best ring rotation, 2^8 - 2^12 blocks.
+NARROW_MIX = [(16, 12, 20, 0, 8), (20, 16, 40, 1, 10), (24, 24, 80, 3, 8), (28, 32, 96, 1, 10),
+              (32, 30, 100, 0, 12), (24, 20, 60, 2, 10), (28, 24, 80, 2, 10), (32, 28, 90, 1, 12)]
+
+
+def narrow_mix(srv, lanes=4, max_batch=4):
+    """Add the eight narrow ring circuits of NARROW_MIX, equally weighted; returns the Poisson mix table."""
+    mix = {}
+    for n, d, t, seed, k in NARROW_MIX:
+        args, w = _synthetic(n, d, t, seed, k)
+        name = f'{n}q/d{d}/{t}T/s{seed}'
+        srv.add(name, *args, lanes=lanes, max_batch=max_batch, schedule='compiled', lds_storage=True)
+        mix[name] = (1, n, args[5])
+    return mix
+
+
+def _serve_narrow_mix(a, log):
+    """The narrow mix served twice from the same seeded stream: one compiled engine and worker per
+    circuit, then every circuit in shared launches (narrow_width 20).  One JSON line."""
+    out = {'metric': 'serving narrow circuits: p50/p99 ms, per-circuit engines vs shared launches',
+           'requests': a.requests, 'rate_per_s': a.rate, 'seed': a.seed, 'concurrent': not a.sequential,
+           'circuits': [f'{n}q/d{d}/{t}T seed {s}, 2^{k} block' for n, d, t, s, k in NARROW_MIX]}
+    for label, width in (('per_circuit', 0), ('shared_launches', 20)):
+        srv = Server(narrow_width=width)
+        mix = narrow_mix(srv)
+        for j, name in enumerate(mix):          # warm-up outside the statistics
+            srv.serve([Request(-1 - j, name, np.zeros(mix[name][1], np.uint8), 0.5, 0.0)], realtime=False)
+        reqs = poisson_workload(mix, a.rate, a.requests, a.seed)
+        t1 = time.perf_counter()
+        srv.serve(reqs, realtime=True, concurrent=not a.sequential)
+        span = time.perf_counter() - t1
+        st = latency_stats(reqs)['all']
+        out[label] = {'serve_s': span, 'samples_per_s': len(reqs) / span, 'p50_ms': st['p50_ms'],
+                      'p99_ms': st['p99_ms'], 'mean_ms': st['mean_ms'], 'shared_batches': srv.narrow_batches}
+        log(label, out[label])
+        srv.close()
+    print(json.dumps(out), flush=True)
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description='Serving benchmark: Poisson mix of ring circuits on one GPU')
@@ -448,8 +597,13 @@ def main(argv=None):
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--no-2-22', action='store_true', help='64-qubit requests with 2^20 blocks only')
     ap.add_argument('--sequential', action='store_true', help='one loop instead of one worker thread per circuit')
+    ap.add_argument('--narrow-mix', action='store_true',
+                    help='instead: eight narrow circuits, per-circuit engines against shared launches')
     a = ap.parse_args(argv)
     log = lambda *m: print(*m, file=__import__('sys').stderr, flush=True)
+    if a.narrow_mix:
+        _serve_narrow_mix(a, log)
+        return
     srv = Server()
     t0 = time.perf_counter()
     mix, widths = c5_mix(srv, with_2_22=not a.no_2_22, log=log)

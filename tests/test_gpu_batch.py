@@ -98,3 +98,39 @@ def test_batch_rejects_mixed_contexts():
     for e in (ea, eb, ec):
         e.close()
     ctx.close()
+
+
+@pytest.mark.parametrize('concurrent', [False, True])
+def test_serving_narrow_circuits_in_shared_launches(concurrent):
+    """Server(narrow_width=20): four narrow circuits served through shared launches; every request's block
+    equals its circuit's fused engine alone, bit for bit, and its draw equals draw_checked on that block."""
+    from dcs_amd.serving import Request, Server, latency_stats
+    from test_gpu_serving import assert_draw_matches_host
+    print(f'case narrow serving concurrent={concurrent}', flush=True)
+    srv = Server(hbm_budget=16 << 30, narrow_width=20)
+    specs = {f'q{s[0]}w{i}': _engine_args(*s) for i, s in enumerate(SPECS[:5]) if i != 1}
+    for name, args in specs.items():
+        srv.add(name, *args, lanes=2, max_batch=2)
+    rng = np.random.default_rng(2)
+    names = list(specs)
+    reqs = []
+    for i in range(24):
+        name = names[int(rng.integers(len(names)))]
+        x = _prefixes(specs[name], 1, 100 + i)[0]
+        reqs.append(Request(i, name, x, float(rng.random()), 0.002 * (i // 3)))
+    srv.serve(reqs, realtime=concurrent, keep_blocks=True, concurrent=concurrent)
+    assert srv.narrow_batches >= 1 and all(c.narrow for c in srv.circuits.values())
+    print('narrow batches', srv.narrow_batches, latency_stats(reqs)['all'], flush=True)
+    alone = {}
+    for name, args in specs.items():
+        e = InplaceConditionalBlockGPU(*args)
+        e.alloc(dtype=np.complex64, schedule='fused')
+        alone[name] = e
+    for r in reqs:
+        v, _ = alone[r.circuit].block(r.x_bits)
+        ref = np.asarray(v)
+        assert np.array_equal(ref.view(np.float64), r.block.view(np.float64)), r.rid
+        assert_draw_matches_host(r.draw, r.block, r.uniform)
+    for e in alone.values():
+        e.close()
+    srv.close()

@@ -183,6 +183,80 @@ def test_requests_admit_circuits_as_memory_frees(concurrent):
     assert srv.used == 0
 
 
+class _FakeCtx:
+    made = 0
+
+    def __init__(self, device):
+        _FakeCtx.made += 1
+        self.closed = False
+
+    def close(self):
+        self.closed = True
+
+
+class NarrowEngine(TrackedEngine):
+    """Stand-in recording how it was allocated (schedule, shared context)."""
+
+    def alloc(self, dtype, device, lanes, blocks, **kw):
+        super().alloc(dtype, device, lanes, blocks, **kw)
+        self.kw = kw
+
+
+def _fake_batch(log):
+    import time
+
+    def batch(work):
+        engines = {id(e) for e, _, _ in work}
+        for e, _, b in work:
+            assert id(e) in TrackedEngine.live and 0 <= b < e.blocks
+        log.append(sorted(e.inplace_rmax for e in {id(e): e for e, _, _ in work}.values()))
+        time.sleep(0.002 * len(engines))
+        return [_Vec(x) for _, x, _ in work], {'launches': 1}
+    return batch
+
+
+@pytest.mark.parametrize('concurrent', [False, True])
+def test_narrow_circuits_share_batches(concurrent):
+    """Server(narrow_width=w): circuits of width <= w are allocated in the fused schedule on one shared
+    context and served together (one batch takes the waiting requests of all of them); wider circuits
+    keep their own batches; results, FIFO order and the budget hold, and a narrow circuit is evicted
+    and re-admitted like any other."""
+    log = []
+    _FakeCtx.made = 0
+    per_lane = footprint_bytes(_BP(2, 30), np.complex64, 1, 2)
+    small = footprint_bytes(_BP(2, 11), np.complex64, 4, 2)
+    srv = Server(hbm_budget=per_lane + 2 * small + (1 << 20), engine_factory=NarrowEngine, narrow_width=12,
+                 batch_fn=_fake_batch(log), context_factory=_FakeCtx)
+    TrackedEngine.server, TrackedEngine.peak_used, TrackedEngine.live = srv, 0, set()
+    SlowEngine.log, SlowEngine.active, SlowEngine.peak = [], 0, 0
+    for name, w in (('n10', 10), ('n11', 11), ('n12', 12)):
+        srv.add(name, w, None, 8, None, [], [0, 1], lanes=4, max_batch=2, schedule='compiled', lds_buffers=3)
+    srv.add('wide', 30, None, 8, None, [], [0, 1], lanes=1, max_batch=2)
+    rng = np.random.default_rng(11)
+    names = ['n10', 'n11', 'wide', 'n10', 'n11', 'n10', 'n12', 'wide', 'n11', 'n12', 'n10', 'n11', 'n12', 'wide']
+    # requests arrive in pairs, so that two narrow circuits often wait at the same time
+    reqs = [Request(i, nm, rng.integers(0, 2, 8, dtype=np.uint8), float(rng.random()), 0.004 * (i // 2))
+            for i, nm in enumerate(names)]
+    want = {r.rid: r.x_bits.sum() * 1000 + int(r.uniform * 1000) for r in reqs}
+    srv.serve(reqs, realtime=concurrent, concurrent=concurrent)
+    assert all(r.index == want[r.rid] and r.done >= r.arrival for r in reqs)
+    assert srv.narrow_batches == len(log) >= 1
+    assert any(len(g) >= 2 for g in log)                 # at least one batch shared by two narrow circuits
+    assert all(w <= 12 for g in log for w in g)          # the wide circuit never joins one
+    assert all(e == 30 for e, _ in SlowEngine.log)      # block_many served only the wide circuit
+    for nm in ('n10', 'n11', 'n12'):
+        kw = srv.catalog[nm].engine.kw
+        assert kw['schedule'] == 'fused' and kw['context'] is srv._shared_ctx and 'lds_buffers' not in kw
+        seq = [r.batch for r in sorted(reqs, key=lambda r: r.arrival) if r.circuit == nm]
+        assert seq == sorted(seq)
+    assert srv.catalog['wide'].engine.kw.get('context') is None
+    assert _FakeCtx.made == 1
+    assert TrackedEngine.peak_used <= srv.budget and srv.evictions >= 1
+    ctx = srv._shared_ctx
+    srv.close()
+    assert ctx.closed and srv.used == 0
+
+
 def test_poisson_workload_is_seeded_and_weighted():
     mix = {'a': (3, 10, [8, 9]), 'b': (1, 12, [0])}
     r1 = poisson_workload(mix, rate=50.0, count=400, seed=7)
