@@ -56,21 +56,33 @@ def prefixes(count):
 # product; 3 replace: f0 (a + b) and f1 (a +- b) per pair; 5: f (a +- b) per pair; 6 shrink: two outputs
 # f0 (a + b) + f1 (e +- g) per quad; 7: W0 a + W1 b per pair.
 FLOPS_PER_INPUT = {0: 12, 1: 14, 2: 6, 3: 8, 4: 6, 5: 4, 6: 9, 7: 7}
+# The same for an f0-factored step (coefficients (1, r), r = f1 / f0) whose ratio r is real or imaginary
+# (a product by r is 2 flops, a + r b is 4) and for a complex r (6 and 8): 0: r a; 1: a + r b per output;
+# 2: x + r x; 3: a + b and r (a +- b) per pair; 4: r x; 5: r (a +- b) per pair; 6: (a + b) + r (e +- g) per
+# output of a quad; 7: (a + b) + r (a +- b) per pair.
+FLOPS_PER_INPUT_FACTORED = {1: {0: 2, 1: 4, 2: 4, 3: 3, 4: 2, 5: 2, 6: 4, 7: 4},
+                            3: {0: 6, 1: 8, 2: 8, 3: 5, 4: 6, 5: 4, 6: 6, 7: 6}}
 FP64_PEAK = 78.6e12          # MI355X FP64 vector, FLOP/s
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4   # wave64 VALU instructions per second: 4 SIMDs per CU, 4 cycles each
 
 
-def fused_flops_per_sample(engine):
-    """Algorithmic FP64 flops of the steps inside coset groups, per sample (live inputs only)."""
+def fused_flops_per_sample(engine, factored=False):
+    """Algorithmic FP64 flops of the steps inside coset groups, per sample (live inputs only); with
+    factored, those of the f0-factored kernels (every group's last step and the steps of class 0 keep
+    the unfactored formulas)."""
     bp = engine.bp
     kinds = bp.step_table()['kind']
+    cls = bp.coefficient_classes()
     st = bp.inplace_steps
     total = 0
     for kind, first, count, _ in engine.schedule():
         if kind != 2:
             continue
         for i in range(first, first + count):
-            total += FLOPS_PER_INPUT[int(kinds[i])] << len(st[i].active_in)
+            k = int(kinds[i])
+            c = int(cls[i]) if factored and i < first + count - 1 else 0
+            per = FLOPS_PER_INPUT[k] if c == 0 else FLOPS_PER_INPUT_FACTORED[3 if c == 3 else 1][k]
+            total += per << len(st[i].active_in)
     return total << len(bp.slices)
 
 
@@ -335,6 +347,7 @@ def main():
     ap.add_argument('--schedule', default='compiled', choices=['compiled', 'fused', 'steps', 'steps_host'])
     ap.add_argument('--lds-fp64', action='store_true', help='compiled schedule: FP64 LDS tiles')
     ap.add_argument('--lds-buffers', type=int, default=None, help='compiled schedule: tile pipeline (engine default if unset)')
+    ap.add_argument('--no-factor', action='store_true', help='compiled schedule: group kernels without f0 factoring')
     ap.add_argument('--no-graphs', action='store_true',
                     help='launch every kernel of a slice directly instead of replaying its captured HIP graph')
     ap.add_argument('--lanes', type=int, default=2,
@@ -369,7 +382,7 @@ def main():
     log(f'stream copy (best launch shape): {stream_GBps:.1f} GB/s')
 
     log('fast path: complex64, k2 plan, schedule', a.schedule)
-    kw = {'lds_storage': not a.lds_fp64} if a.schedule == 'compiled' else {}
+    kw = {'lds_storage': not a.lds_fp64, 'factor': not a.no_factor} if a.schedule == 'compiled' else {}
     if a.lds_buffers is not None:
         kw['lds_buffers'] = a.lds_buffers
     if a.schedule != 'steps_host':
@@ -385,7 +398,8 @@ def main():
     bp = eng.bp
     unfused_bytes = bp.bytes_per_slice(8) * (1 << len(bp.slices))
     fb = fused_bytes_per_sample(eng, 8)
-    fused_flops = fused_flops_per_sample(eng)
+    factored = eng.factoring()[1] if a.schedule == 'compiled' else False
+    fused_flops = fused_flops_per_sample(eng, factored)
     coset_ms = ms[2]
     coset_bytes = fb[2]
     achieved = coset_bytes / (coset_ms / 1e3) if coset_ms > 0 else None
@@ -455,8 +469,9 @@ def main():
         'config': {'workload': 'IBM QAT #245 64q/d73/314T injected graph (N=378), plan inplace_k2_fp32 '
                                '(4 parity slices, rmax 34, 128 GiB c64 frontier), 2^20 block, 44-bit prefixes',
                    'schedule': a.schedule, 'lds_tiles': ('fp64' if a.lds_fp64 else 'storage') if a.schedule == 'compiled' else None,
-                   'fusion': {k: fast['options'][k] for k in ('fuse_max_dim', 'tile_log2', 'pass_max_dim', 'group_policy',
-                                                               'lds_buffers', 'graphs')},
+                   'fusion': {**{k: fast['options'][k] for k in ('fuse_max_dim', 'tile_log2', 'pass_max_dim',
+                                                                  'group_policy', 'lds_buffers', 'graphs')},
+                              'f0_factored': factored},
                    'batch': fast['lanes'],
                    'block': 1 << 20},
         'setup_s': fast['setup'], 'fast_samples_run': fast['samples_run'],

@@ -20,7 +20,7 @@ MODE_STEPS, MODE_STEPS_HOST, MODE_FUSED, MODE_COMPILED = 0, 1, 2, 3
 EXPORTED = (
     'dcs_last_error', 'dcs_get_device_count', 'dcs_get_device_info', 'dcs_context_create',
     'dcs_context_destroy', 'dcs_synchronize', 'dcs_plan_create', 'dcs_gather_plan_create', 'dcs_plan_destroy',
-    'dcs_plan_compile', 'dcs_plan_kernel_source', 'dcs_plan_memory', 'dcs_plan_use_graphs',
+    'dcs_plan_compile', 'dcs_plan_factoring', 'dcs_plan_kernel_source', 'dcs_plan_memory', 'dcs_plan_use_graphs',
     'dcs_plan_units',
     'dcs_plan_schedule', 'dcs_contract', 'dcs_block_zero', 'dcs_block_accumulate', 'dcs_block_transform',
     'dcs_block_read', 'dcs_block_device_pointer', 'dcs_draw', 'dcs_frontier_read', 'dcs_plan_profile',
@@ -44,7 +44,7 @@ class PlanDesc(C.Structure):
                 ('open_unitaries', C.POINTER(C.c_double)), ('blocks', C.c_int32),
                 ('fuse_max_dim', C.c_int32), ('resident_bits', C.c_int32), ('tile_log2', C.c_int32),
                 ('exact_rounding', C.c_int32), ('pass_max_dim', C.c_int32), ('lds_storage', C.c_int32),
-                ('group_policy', C.c_int32), ('lds_buffers', C.c_int32)]
+                ('group_policy', C.c_int32), ('lds_buffers', C.c_int32), ('coef_class', C.POINTER(C.c_int32))]
 
 
 class BatchItem(C.Structure):
@@ -98,6 +98,7 @@ def load():
         'dcs_gather_plan_create': (C.c_int, [vp, P(GatherDesc), P(vp)]),
         'dcs_plan_destroy': (None, [vp]),
         'dcs_plan_compile': (C.c_int, [vp, P(dbl)]),
+        'dcs_plan_factoring': (C.c_int, [vp, P(i32), P(i32)]),
         'dcs_plan_kernel_source': (C.c_int, [P(PlanDesc), C.c_char_p, u64, P(u64), P(i32), P(i32), P(i32), P(i32),
                                              C.c_int, P(C.c_int)]),
         'dcs_plan_memory': (C.c_int, [vp, P(u64)]),
@@ -234,8 +235,9 @@ class Context:
 
 def plan_desc(table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, phase, open_unitaries, blocks=1,
               fuse_max_dim=6, resident_bits=12, tile_log2=11, exact_rounding=False, pass_max_dim=3, lds_storage=False,
-              group_policy=0, lds_buffers=2):
-    """dcs_plan_desc of an in-place step table and the arrays its pointers refer to (keep them alive)."""
+              group_policy=0, lds_buffers=2, coef_class=None):
+    """dcs_plan_desc of an in-place step table and the arrays its pointers refer to (keep them alive).
+    coef_class: per-step class of f1 / f0 for the f0-factored group kernels (None: off)."""
     n = len(table['kind'])
     keep = {}
 
@@ -265,6 +267,11 @@ def plan_desc(table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, p
     d.pass_max_dim = int(pass_max_dim)
     d.lds_storage = int(bool(lds_storage))
     d.group_policy, d.lds_buffers = int(group_policy), int(lds_buffers)
+    if coef_class is not None:
+        keep['coef_class'] = np.ascontiguousarray(coef_class, dtype=np.int32)
+        if keep['coef_class'].shape != (n,):
+            raise ValueError(f'coef_class needs one entry per step ({n})')
+        d.coef_class = ptr(keep['coef_class'], C.c_int32)
     return d, keep
 
 
@@ -291,12 +298,12 @@ class Plan:
 
     def __init__(self, ctx, table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, phase,
                  open_unitaries, blocks=1, fuse_max_dim=6, resident_bits=12, tile_log2=11, exact_rounding=False,
-                 pass_max_dim=3, lds_storage=False, group_policy=0, lds_buffers=2):
+                 pass_max_dim=3, lds_storage=False, group_policy=0, lds_buffers=2, coef_class=None):
         self.ctx, self.lib = ctx, ctx.lib
         n = len(table['kind'])
         d, self._keep = plan_desc(table, maxslots, storage, lanes, k, terminal_mask, syndrome_map, phase,
                                   open_unitaries, blocks, fuse_max_dim, resident_bits, tile_log2, exact_rounding,
-                                  pass_max_dim, lds_storage, group_policy, lds_buffers)
+                                  pass_max_dim, lds_storage, group_policy, lds_buffers, coef_class)
         h = C.c_void_p()
         check(self.lib.dcs_plan_create(ctx.handle, C.byref(d), C.byref(h)))
         self.handle = h
@@ -352,6 +359,12 @@ class Plan:
         sec = C.c_double(0.0)
         check(self.lib.dcs_plan_compile(self.handle, C.byref(sec)))
         return sec.value
+
+    def factoring(self):
+        """(module factored, last compiled contraction ran the factored kernels) as booleans."""
+        a, b = C.c_int32(0), C.c_int32(0)
+        check(self.lib.dcs_plan_factoring(self.handle, C.byref(a), C.byref(b)))
+        return bool(a.value), bool(b.value)
 
     def use_graphs(self, enable=True):
         """Replay contract()'s launch sequence from captured HIP graphs (one per mode and lane count)."""

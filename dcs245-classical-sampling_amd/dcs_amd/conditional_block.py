@@ -248,7 +248,8 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
               fuse_max_dim=None, resident_bits=12, tile_log2=None, blocks=1, exact_rounding=False,
-              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=3, graphs=False, context=None):
+              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=3, graphs=False, context=None,
+              factor=True):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:85-105).
 
         dtype      np.complex64 (c64 storage, FP64 arithmetic) or np.complex128
@@ -276,6 +277,12 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    a group allows it (complex64 tiles in LDS, at least two lanes), else as 1; the same
                    arithmetic as 1, so the same blocks bit for bit; 4 as 3 with the last register pass
                    storing to HBM from registers and the scale folded into the group's first step
+        factor     compiled schedule with complex64 tiles rounded once per register pass (lds_buffers
+                   1..3): run every step whose f0 cannot be zero with the coefficients (1, f1 / f0) and
+                   apply the pending f0s at each group's last step (BlockPlan.coefficient_classes says
+                   whether f1 / f0 is real, imaginary or complex, which sets the FP64 work of the step);
+                   a slice whose parameters do not fit runs the unfactored kernels; an array gives the
+                   per-step classes instead of the plan's own
         graphs     replay each slice's launch sequence from a captured HIP graph (one per lane count)
         context    a _native.Context to share (engines contracted together by block_batch must share
                    one); the engine then leaves it open on close().  Default: a context of its own
@@ -305,16 +312,29 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                                   blocks=blocks, fuse_max_dim=fuse_max_dim, resident_bits=resident_bits,
                                   tile_log2=tile_log2, exact_rounding=exact_rounding,
                                   pass_max_dim=pass_max_dim, lds_storage=lds_storage,
-                                  group_policy=group_policy, lds_buffers=lds_buffers)
+                                  group_policy=group_policy, lds_buffers=lds_buffers,
+                                  coef_class=self._coef_classes(factor))
         self.schedule_options = {'schedule': schedule, 'fuse_max_dim': fuse_max_dim, 'tile_log2': tile_log2,
                                  'pass_max_dim': pass_max_dim, 'resident_bits': resident_bits,
                                  'group_policy': group_policy, 'lds_buffers': lds_buffers,
-                                 'lds_storage': bool(lds_storage), 'exact_rounding': bool(exact_rounding)}
+                                 'lds_storage': bool(lds_storage), 'exact_rounding': bool(exact_rounding),
+                                 'factor': not (factor is False or factor is None)}
         self.compile_seconds = self._plan.compile() if self.mode == _native.MODE_COMPILED else 0.0
         if graphs:
             self._plan.use_graphs(True)
         self.schedule_options['graphs'] = bool(graphs)
         self.buf = True
+
+    def _coef_classes(self, factor):
+        if factor is False or factor is None:
+            return None
+        if factor is True:
+            return self.bp.coefficient_classes()
+        return np.asarray(factor, dtype=np.int32)
+
+    def factoring(self):
+        """(group kernels f0-factored, the last contraction ran them): see alloc(factor=...)."""
+        return self._plan.factoring()
 
     # ---- reference mixin hooks -------------------------------------------------------------
     def prefix_inplace(self, f, yfix, lane=0):

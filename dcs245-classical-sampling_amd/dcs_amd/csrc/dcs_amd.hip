@@ -944,6 +944,15 @@ struct dcs_plan {
     std::vector<hipFunction_t> group_fn;
     std::vector<unsigned> group_grid;
     std::vector<int> group_threads;
+    // f0 factoring of the group kernels: per-step coefficient classes (empty: off); when on, group_fn holds
+    // the factored kernels and group_fn_plain the unfactored ones, compiled when a contraction's
+    // parameters first do not fit the classes; call_factored says which set the staged call runs
+    std::vector<int8_t> coef_class;
+    bool factored_module = false;
+    bool compiled_plain = false;
+    std::vector<hipFunction_t> group_fn_plain;
+    std::vector<unsigned> group_grid_plain;
+    bool call_factored = false;
     int units_steps, units_fused;
     // device memory
     void *frontier = nullptr;
@@ -971,7 +980,7 @@ struct dcs_plan {
     double prof_ms[5] = {0, 0, 0, 0, 0};
     int64_t prof_n[5] = {0, 0, 0, 0, 0};
     std::vector<PendingEvent> pending;
-    // captured launch sequences of dcs_contract, one per (mode, lane count)
+    // captured launch sequences of dcs_contract, one per (mode, lane count, factored kernels or not)
     bool use_graphs = false;
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
 };
@@ -1517,10 +1526,21 @@ ModuleCache &module_cache() {
     return c;
 }
 
+// Codegen options of the plan's group kernels; with factored, the f0 factoring of the plan's coefficient
+// classes where the schedule allows it.
+CodegenOptions codegen_options(const dcs_plan *p, bool factored) {
+    CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0, p->lds_buffers};
+    if (factored && !p->coef_class.empty()) opt.cls = p->coef_class.data();
+    return opt;
+}
+
+// Whether the plan's compiled schedule runs f0-factored group kernels by default.
+bool factoring_on(const dcs_plan *p) { return codegen_options(p, true).factored(); }
+
 // Source of the module holding one generated kernel per fused group of the plan's schedule.
 std::string group_module_source(const dcs_plan *p, const std::vector<GroupDev> &groups, const std::vector<PassDev> &passes,
-                                const std::vector<PStepDev> &psteps, const std::vector<u64> &addrv) {
-    CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0, p->lds_buffers};
+                                const std::vector<PStepDev> &psteps, const std::vector<u64> &addrv, bool factored) {
+    const CodegenOptions opt = codegen_options(p, factored);
     std::string src = codegen::prelude(opt, kKernelsHpp);
     for (size_t gi = 0; gi < groups.size(); ++gi) {
         const GroupDev &g = groups[gi];
@@ -1530,11 +1550,11 @@ std::string group_module_source(const dcs_plan *p, const std::vector<GroupDev> &
     return src;
 }
 
-int compile_groups(dcs_plan *p, double *seconds) {
-    if (seconds) *seconds = 0.0;
-    if (p->executor != 0) return fail(DCS_ERR_INVALID, "a gather plan has no coset groups to compile");
-    if (p->compiled) return DCS_OK;
-    const std::string src = group_module_source(p, p->h_groups, p->h_passes, p->h_psteps, p->h_addrv);
+// Compile (or find in the module cache) the plan's group kernels, factored or not, and size their
+// persistent grids from their occupancy.
+int compile_module(dcs_plan *p, bool factored, std::vector<hipFunction_t> &fn, std::vector<unsigned> &grid,
+                   double *seconds) {
+    const std::string src = group_module_source(p, p->h_groups, p->h_passes, p->h_psteps, p->h_addrv, factored);
     HIPCHK(hipSetDevice(p->ctx->device));
     hipModule_t mod = nullptr;
     {
@@ -1568,23 +1588,42 @@ int compile_groups(dcs_plan *p, double *seconds) {
             hiprtcDestroyProgram(&prog);
             HIPCHK(hipModuleLoadData(&mod, code.data()));
             mc.mods.emplace(std::move(key), mod);
-            if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (seconds) *seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
     }
-    p->group_fn.assign(p->h_groups.size(), nullptr);
-    p->group_grid.assign(p->h_groups.size(), 1);
+    fn.assign(p->h_groups.size(), nullptr);
+    grid.assign(p->h_groups.size(), 1);
     p->group_threads.assign(p->h_groups.size(), kThreads);
     for (size_t gi = 0; gi < p->h_groups.size(); ++gi) {
         const std::string name = "dcs_group_" + std::to_string(gi);
-        HIPCHK(hipModuleGetFunction(&p->group_fn[gi], mod, name.c_str()));
+        HIPCHK(hipModuleGetFunction(&fn[gi], mod, name.c_str()));
         p->group_threads[gi] = codegen::group_threads(p->h_groups[gi]);
         int nb = 0;
-        HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, p->group_fn[gi], p->group_threads[gi], 0));
+        HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn[gi], p->group_threads[gi], 0));
         if (nb < 1) return fail(DCS_ERR_UNSUPPORTED, "group kernel %zu cannot be resident (registers or LDS)", gi);
         const u64 cap = (u64)p->ctx->cus * (u64)std::max(1, nb);
-        p->group_grid[gi] = (unsigned)std::max<u64>(1, std::min<u64>(p->h_groups[gi].ntiles, cap));
+        grid[gi] = (unsigned)std::max<u64>(1, std::min<u64>(p->h_groups[gi].ntiles, cap));
     }
+    return DCS_OK;
+}
+
+int compile_groups(dcs_plan *p, double *seconds) {
+    if (seconds) *seconds = 0.0;
+    if (p->executor != 0) return fail(DCS_ERR_INVALID, "a gather plan has no coset groups to compile");
+    if (p->compiled) return DCS_OK;
+    p->factored_module = factoring_on(p);
+    const int rc = compile_module(p, p->factored_module, p->group_fn, p->group_grid, seconds);
+    if (rc) return rc;
     p->compiled = true;
+    return DCS_OK;
+}
+
+// The unfactored group kernels of a plan whose default kernels are f0-factored (compiled on first use).
+int compile_plain(dcs_plan *p) {
+    if (p->compiled_plain) return DCS_OK;
+    const int rc = compile_module(p, false, p->group_fn_plain, p->group_grid_plain, nullptr);
+    if (rc) return rc;
+    p->compiled_plain = true;
     return DCS_OK;
 }
 
@@ -1597,7 +1636,9 @@ int launch_compiled(dcs_plan *p, int nl, const Seg &sg, int nunits) {
     int nsteps = p->nsteps, nu = nunits, unit = sg.unit;
     u64 *dmax = p->d_dmax;
     void *args[] = {&E, &ls, &lanes, &dyn, &nsteps, &dmax, &nu, &unit};
-    PROF(2, HIPCHK(hipModuleLaunchKernel(p->group_fn[sg.group], p->group_grid[sg.group], (unsigned)nl, 1,
+    const bool fac = !p->factored_module || p->call_factored;
+    PROF(2, HIPCHK(hipModuleLaunchKernel(fac ? p->group_fn[sg.group] : p->group_fn_plain[sg.group],
+                                         fac ? p->group_grid[sg.group] : p->group_grid_plain[sg.group], (unsigned)nl, 1,
                                          (unsigned)p->group_threads[sg.group], 1, 1, 0,
                                          p->ctx->stream, args, nullptr)));
     return DCS_OK;
@@ -1682,6 +1723,54 @@ int check_lanes(const dcs_plan *p, int nl, const int32_t *lanes) {
     return DCS_OK;
 }
 
+// The g0, g1 parameters of the f0-factored group kernels for the staged lanes.  Within each coset group
+// a step of class c > 0 (not the last) gets g1 = f1 / f0 (the part its class uses) and leaves f0
+// pending; the last step gets (g0, g1) = (P f0, P f1), P the product of the pending f0s in step order.
+// False when some lane does not fit the classes: f0 == 0, a ratio not of its class (relative 1e-12)
+// or beyond 2^30, or P not finite and nonzero.
+bool stage_factors(dcs_plan *p, int nl) {
+    const int n = p->nsteps;
+    for (int li = 0; li < nl; ++li) {
+        Dyn *dl = p->h_dyn + (size_t)li * n;
+        for (const Seg &sg : p->sched_fused) {
+            if (sg.type != 2) continue;
+            double Pr = 1.0, Pi = 0.0;
+            const int last = sg.first + sg.count - 1;
+            for (int s = sg.first; s < last; ++s) {
+                const int c = p->coef_class[s];
+                if (!c) continue;
+                Dyn &d = dl[s];
+                const double m2 = d.f0r * d.f0r + d.f0i * d.f0i;
+                if (!(m2 > 0.0) || !std::isfinite(m2)) return false;
+                const double rr = (d.f1r * d.f0r + d.f1i * d.f0i) / m2, ri = (d.f1i * d.f0r - d.f1r * d.f0i) / m2;
+                const double mag = std::hypot(rr, ri);
+                if (!std::isfinite(mag) || mag > 0x1p30) return false;
+                if (c == 1) {
+                    if (std::fabs(ri) > 1e-12 * mag) return false;
+                    d.g1r = rr;
+                } else if (c == 2) {
+                    if (std::fabs(rr) > 1e-12 * mag) return false;
+                    d.g1i = ri;
+                } else {
+                    d.g1r = rr;
+                    d.g1i = ri;
+                }
+                const double nr = Pr * d.f0r - Pi * d.f0i, ni = Pr * d.f0i + Pi * d.f0r;
+                Pr = nr;
+                Pi = ni;
+            }
+            const double pm = std::hypot(Pr, Pi);
+            if (!(pm > 0.0) || !std::isfinite(pm)) return false;
+            Dyn &d = dl[last];
+            d.g0r = Pr * d.f0r - Pi * d.f0i;
+            d.g0i = Pr * d.f0i + Pi * d.f0r;
+            d.g1r = Pr * d.f1r - Pi * d.f1i;
+            d.g1i = Pr * d.f1i + Pi * d.f1r;
+        }
+    }
+    return true;
+}
+
 // Per-lane step parameters and lane ids into the plan's pinned staging buffers.
 void stage_parameters(dcs_plan *p, int nl, const int32_t *lanes, const double *coeffs, const int32_t *ov,
                       const int32_t *y) {
@@ -1697,8 +1786,10 @@ void stage_parameters(dcs_plan *p, int nl, const int32_t *lanes, const double *c
             d.ov = ov[(size_t)li * n + s] & 1;
             d.y = y[(size_t)li * n + s] & 1;
             d.pad[0] = d.pad[1] = 0;
+            d.g0r = d.g0i = d.g1r = d.g1i = 0.0;
         }
     for (int i = 0; i < nl; ++i) p->h_lanes[i] = lanes[i];
+    p->call_factored = factoring_on(p) && stage_factors(p, nl);
 }
 
 // One launch of dcs_contract_batch: entries [first, first + count) of the round, all of one segment type.
@@ -1943,6 +2034,13 @@ int ingest_plan_steps(const dcs_plan_desc *d, dcs_plan *p, std::vector<StepDev> 
     p->lds_storage = d->lds_storage ? 1 : 0;
     p->group_policy = d->group_policy;
     p->lds_buffers = d->lds_buffers ? d->lds_buffers : 2;
+    p->coef_class.clear();
+    if (d->coef_class) {
+        for (int s = 0; s < d->nsteps; ++s)
+            if (d->coef_class[s] < 0 || d->coef_class[s] > 3)
+                return fail(DCS_ERR_INVALID, "coef_class[%d] = %d outside 0..3", s, d->coef_class[s]);
+        p->coef_class.assign(d->coef_class, d->coef_class + d->nsteps);
+    }
     p->esz = storage_bytes(d->storage);
     p->lane_stride = 1ull << d->maxslots;
     p->terminal_mask = d->terminal_mask;
@@ -2217,6 +2315,13 @@ int dcs_plan_compile(dcs_plan *p, double *seconds) {
     return compile_groups(p, seconds);
 }
 
+int dcs_plan_factoring(dcs_plan *p, int32_t *module_factored, int32_t *last_call_factored) {
+    if (!p || !module_factored || !last_call_factored) return fail(DCS_ERR_INVALID, "null argument");
+    *module_factored = p->factored_module ? 1 : 0;
+    *last_call_factored = (p->factored_module && p->call_factored) ? 1 : 0;
+    return DCS_OK;
+}
+
 int dcs_plan_kernel_source(const dcs_plan_desc *d, char *src, uint64_t src_cap, uint64_t *src_len, int32_t *group,
                            int32_t *first, int32_t *count, int32_t *threads, int cap, int *ngroups) {
     if (!d || !src_len || !ngroups) return fail(DCS_ERR_INVALID, "null argument");
@@ -2235,7 +2340,7 @@ int dcs_plan_kernel_source(const dcs_plan_desc *d, char *src, uint64_t src_cap, 
         if (!rc) rc = build_schedules(p.get(), groups, passes, psteps, addrv);
         if (rc) return rc;
     }
-    const std::string text = group_module_source(p.get(), groups, passes, psteps, addrv);
+    const std::string text = group_module_source(p.get(), groups, passes, psteps, addrv, true);
     *src_len = text.size();
     if (src && src_cap) {
         const size_t n = std::min<size_t>(text.size(), (size_t)src_cap - 1);
@@ -2301,6 +2406,10 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
         if (rc) return rc;
     }
     stage_parameters(p, nl, lanes, coeffs, ov, y);
+    if (mode == DCS_MODE_COMPILED && p->factored_module && !p->call_factored) {
+        const int rc = compile_plain(p);
+        if (rc) return rc;
+    }
     hipStream_t st = p->ctx->stream;
     const int nunits_out = (mode >= DCS_MODE_FUSED) ? p->units_fused : p->units_steps;
     // Graph replay: the whole sequence below (parameter upload from the pinned staging buffers, maxima
@@ -2308,7 +2417,9 @@ int dcs_contract(dcs_plan *p, int mode, int nl, const int32_t *lanes, const doub
     // (mode, lane count) and replayed; the staging buffers are read when a replay runs.
     const bool graphable = p->use_graphs && !p->profiling && p->executor == 0 && mode != DCS_MODE_STEPS_HOST;
     if (graphable) {
-        const auto key = std::make_pair(mode, nl);
+        // the factored and unfactored group kernels are captured apart
+        const bool plain = mode == DCS_MODE_COMPILED && p->factored_module && !p->call_factored;
+        const auto key = std::make_pair(mode + (plain ? 16 : 0), nl);
         auto it = p->graphs.find(key);
         if (it == p->graphs.end()) {
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));

@@ -95,11 +95,14 @@ __device__ __forceinline__ double scale_from_m2(double m2) {
 }
 
 // Per-lane, per-step parameters of a contraction: the step vertex's local vector (f0, f1), the offset
-// parity bit and the forced value of a structurally fixed vertex.
+// parity bit and the forced value of a structurally fixed vertex.  g0, g1 serve the f0-factored group
+// kernels: the ratio f1 / f0 in g1 for a factored step, and (P f0, P f1) for the last step of a group,
+// P the product of the f0s its factored steps left pending.
 struct Dyn {
     double f0r, f0i, f1r, f1i;
     int32_t ov, y;
     int32_t pad[2];
+    double g0r, g0i, g1r, g1i;
 };
 
 __device__ __forceinline__ double bits2d(u64 b) { return __longlong_as_double((long long)b); }

@@ -40,6 +40,13 @@ struct CodegenOptions {
     bool lds_storage;  // LDS tiles in storage precision (rounded once per register pass) instead of FP64
     int lds_buffers;   // LDS tiles per workgroup: 2 (double-buffered) or 1; 3: two tiles filled by
                        // asynchronous global-to-LDS loads where a group allows it (else as 1)
+    // Per-step coefficient classes for f0-factored steps (nullptr: none): 0 the step runs with (f0, f1);
+    // 1, 2, 3 the ratio f1 / f0 is real, imaginary or complex and the step runs with (1, f1 / f0).  Used
+    // only where factored() holds.
+    const int8_t *cls = nullptr;
+    // f0 factoring: complex64 storage rounded once per register pass (the FP64 arithmetic between roundings
+    // is free to change), the scale applied at the store (not folded into a step).
+    bool factored() const { return cls && storage == 0 && !exact && lds_storage && lds_buffers != 4; }
 };
 
 namespace codegen {
@@ -110,6 +117,20 @@ __device__ __forceinline__ cd cfma2(cd p, cd a, cd q, cd b) {
     i = fma(p.i, a.r, i);
     return cd{r, i};
 }
+// f0-factored steps: p is the lane-signed ratio f1 / f0 of a step whose class C says which parts are
+// nonzero (1: real, 2: imaginary, 3: both); rmul<C>(p, b) = p b and radd<C, U>(a, p, b) = a + (-1)^U p b,
+// one fused multiply-add per component for C 1 and 2
+template <int C> __device__ __forceinline__ cd rmul(cd p, cd b) {
+    if constexpr (C == 1) return cd{p.r * b.r, p.r * b.i};
+    else if constexpr (C == 2) return cd{-(p.i * b.i), p.i * b.r};
+    else return cmul(p, b);
+}
+template <int C, int U> __device__ __forceinline__ cd radd(cd a, cd p, cd b) {
+    const double kr = U ? -p.r : p.r, ki = U ? -p.i : p.i;
+    if constexpr (C == 1) return cd{fma(kr, b.r, a.r), fma(kr, b.i, a.i)};
+    else if constexpr (C == 2) return cd{fma(-ki, b.i, a.r), fma(ki, b.r, a.i)};
+    else return cd{fma(-ki, b.i, fma(kr, b.r, a.r)), fma(ki, b.r, fma(kr, b.i, a.i))};
+}
 // NaN-propagating maximum (llvm.maximum: one v_maximum3_f32 on gfx950)
 __device__ __forceinline__ float max_nan_f(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 )";
@@ -161,7 +182,7 @@ inline u64 lane_dep_const(const GroupDev &g, uint32_t l) {
 // skipped, the maximum |stored|^2 kept) instead of back into the LDS tile; gb<m> / dlo<m> are the
 // per-thread address and liveness values set up by emit_direct_setup.
 inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView &v, const PassDev &ps, int threads,
-                      uint32_t m0 = 0, bool fold = false, bool direct = false) {
+                      uint32_t m0 = 0, bool fold = false, bool direct = false, bool last = false) {
     const GroupDev &g = *v.g;
     const int J = ps.nmask, NE = 1 << J, WB = g.wbits, D = g.d;
     const uint32_t norb = (1u << (D - J)) << WB;
@@ -199,12 +220,23 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
     }
     for (int s = 0; s < ps.nsteps; ++s) {
         const PStepDev &q = v.psteps[ps.first_pstep + s];
-        emitf(o, "    { // step %d kind %d\n", q.step, q.kind);
+        // f0 factoring: a step of class C > 0 (not the group's last) applies (1, rho), rho = f1 / f0, and
+        // leaves the factor f0 pending; the pending factors of a group are one complex number per lane
+        // (they multiply every entry alike), so the group's last step applies them at once with the
+        // coefficients (P f0, P f1), P the product of the pending f0s (g0, g1 of its Dyn entry)
+        const bool fac = opt.factored();
+        const bool fin = fac && last && s + 1 == ps.nsteps;
+        const int C = (fac && !fin) ? opt.cls[q.step] : 0;
+        emitf(o, "    { // step %d kind %d%s\n", q.step, q.kind, C ? " (f0 factored)" : fin ? " (group's pending factors)" : "");
         emitf(o, "      const Dyn &q = dl[%d];\n", q.step);
         const bool sc = fold && s == 0;
-        if (sc) emitf(o, "      const double q_f1r = q.f1r * scale, q_f1i = q.f1i * scale;\n");
-        else emitf(o, "      const double q_f1r = q.f1r, q_f1i = q.f1i;\n");
-        emitf(o, sc ? "      const cd f0{q.f0r * scale, q.f0i * scale};\n" : "      const cd f0{q.f0r, q.f0i};\n");
+        if (C == 0) {
+            if (sc) emitf(o, "      const double q_f1r = q.f1r * scale, q_f1i = q.f1i * scale;\n");
+            else if (fin) emitf(o, "      const double q_f1r = q.g1r, q_f1i = q.g1i;\n");
+            else emitf(o, "      const double q_f1r = q.f1r, q_f1i = q.f1i;\n");
+            emitf(o, sc ? "      const cd f0{q.f0r * scale, q.f0i * scale};\n"
+                        : fin ? "      const cd f0{q.g0r, q.g0i};\n" : "      const cd f0{q.f0r, q.f0i};\n");
+        }
         emitf(o, "      const int sg = (__popcll(tb & 0x%llxull) + q.ov) & 1;\n", (unsigned long long)q.beta);
         auto u = [&](int E) { return par32(TE[E] & q.bvec); };
         for (int m = 0; m < OPT; ++m) {
@@ -215,6 +247,13 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
             // +-f1 by flipping the sign bits (bit-identical to negation): no divergent branch around the
             // parameter loads
             emitf(o, "      const long long sm%d = (long long)((unsigned long long)sb%d << 63);\n", m, m);
+            if (C) {
+                // the lane-signed ratio; only the parts its class uses
+                emitf(o, "      const cd P%d{%s, %s};\n", m,
+                      C & 1 ? (std::string("__longlong_as_double(__double_as_longlong(q.g1r) ^ sm") + std::to_string(m) + ")").c_str() : "0.0",
+                      C & 2 ? (std::string("__longlong_as_double(__double_as_longlong(q.g1i) ^ sm") + std::to_string(m) + ")").c_str() : "0.0");
+                continue;
+            }
             emitf(o, "      const cd F%d{__longlong_as_double(__double_as_longlong(q_f1r) ^ sm%d), "
                      "__longlong_as_double(__double_as_longlong(q_f1i) ^ sm%d)};\n", m, m, m);
             if (q.kind == 2 || q.kind == 7)
@@ -235,7 +274,60 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
                 va_end(ap);
                 ops.push_back({A, std::string(buf)});
             };
-            if (q.kind == 2 || q.kind == 4) {
+            if (C) {
+                // the orbit formulas with (f0, F) replaced by (1, P): W_u = 1 + (-1)^u P, C_u = y ? (-1)^u P : 1,
+                // and f0 a + F b = a + P b
+                if (q.kind == 2 || q.kind == 4) {
+                    for (int E = 0; E < NE; ++E) {
+                        if (q.kind == 2) add(E, "x%d_%d = radd<%d, %d>(x%d_%d, P%d, x%d_%d);", m, E, C, u(E), m, E, m, m, E);
+                        else add(E, "if (q.y) x%d_%d = sgn<%d>(rmul<%d>(P%d, x%d_%d));", m, E, u(E), C, m, m, E);
+                    }
+                } else if (q.kind == 6) {
+                    const uint32_t b1 = q.pat, b2 = q.pat2;
+                    for (int E = 0; E < NE; ++E) {
+                        if ((E & b1) || (E & b2)) continue;
+                        int c = -1;
+                        const int mem[4] = {E, (int)(E ^ b1), (int)(E ^ b2), (int)(E ^ b1 ^ b2)};
+                        for (int X : mem)
+                            if (!par32(X & q.pm) && !par32(X & q.pma)) c = X;
+                        const int c1 = c ^ b1, dd = c ^ b2, d1 = dd ^ b1;
+                        add(c, "{ const cd a = x%d_%d, b = x%d_%d, e = x%d_%d, g = x%d_%d; "
+                               "x%d_%d = radd<%d, 0>(cadd(a, b), P%d, psum<%d, %d>(e, g)); "
+                               "x%d_%d = radd<%d, 0>(cadd(e, g), P%d, psum<%d, %d>(a, b)); }",
+                            m, c, m, c1, m, dd, m, d1, m, c, C, m, u(dd), u(d1), m, dd, C, m, u(c), u(c1));
+                    }
+                } else {
+                    const uint32_t PI = q.pat;
+                    const int tp = top32(PI);
+                    for (int E = 0; E < NE; ++E) {
+                        if ((E >> tp) & 1) continue;
+                        const int EP = E ^ PI;
+                        const int A = par32(E & q.pm) ? EP : E, B = A ^ PI;
+                        switch (q.kind) {
+                            case 0:
+                                add(A, "x%d_%d = sgn<%d>(rmul<%d>(P%d, x%d_%d));", m, B, u(A), C, m, m, A);
+                                break;
+                            case 1:
+                                add(A, "{ const cd a = x%d_%d, b = x%d_%d; x%d_%d = radd<%d, %d>(a, P%d, b); "
+                                       "x%d_%d = radd<%d, %d>(b, P%d, a); }",
+                                    m, A, m, B, m, A, C, u(B), m, m, B, C, u(A), m);
+                                break;
+                            case 3:
+                                add(A, "{ const cd a = x%d_%d, b = x%d_%d; x%d_%d = cadd(a, b); x%d_%d = rmul<%d>(P%d, psum<%d, %d>(a, b)); }",
+                                    m, A, m, B, m, A, m, B, C, m, u(A), u(B));
+                                break;
+                            case 5:
+                                add(A, "{ const cd a = x%d_%d, b = x%d_%d; x%d_%d = q.y ? rmul<%d>(P%d, psum<%d, %d>(a, b)) : cadd(a, b); }",
+                                    m, A, m, B, m, A, C, m, u(A), u(B));
+                                break;
+                            default:  // 7
+                                add(A, "{ const cd a = x%d_%d, b = x%d_%d; x%d_%d = radd<%d, 0>(cadd(a, b), P%d, psum<%d, %d>(a, b)); }",
+                                    m, A, m, B, m, A, C, m, u(A), u(B));
+                                break;
+                        }
+                    }
+                }
+            } else if (q.kind == 2 || q.kind == 4) {
                 for (int E = 0; E < NE; ++E)
                     add(E, "x%d_%d = cmul(x%d_%d, %c%d_%d);", m, E, m, E, q.kind == 2 ? 'W' : 'C', m, u(E));
             } else if (q.kind == 6) {
@@ -619,7 +711,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         o += "  auto passes = [&](Lt *__restrict__ T, Lt *__restrict__ U, bool more, u64 tn) {\n"
              "    if (more) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tile_base(tn)); __builtin_amdgcn_s_setprio(0); }\n";
         for (int pi = 0; pi < g.npasses; ++pi) {
-            emit_pass(o, opt, v, v.passes[pi], threads, m0, pi == 0, pi + 1 == g.npasses);
+            emit_pass(o, opt, v, v.passes[pi], threads, m0, pi == 0, pi + 1 == g.npasses, pi + 1 == g.npasses);
             if (pi + 1 < g.npasses) o += "    dcs_lds_barrier();\n";
         }
         o += "  };\n";
@@ -650,7 +742,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
              "    const u64 tbn = more ? tile_base(tn) : 0ull;\n"
              "    if (more && !prev) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tbn); __builtin_amdgcn_s_setprio(0); }\n";
         for (int pi = 0; pi < g.npasses; ++pi) {
-            emit_pass(o, opt, v, v.passes[pi], threads, m0);
+            emit_pass(o, opt, v, v.passes[pi], threads, m0, false, false, pi + 1 == g.npasses);
             if (pi == 0) {
                 o += "    if (prev) { __builtin_amdgcn_s_setprio(2); store(U, tprev); __builtin_amdgcn_s_setprio(0); }\n"
                      "    dcs_lds_barrier();\n"
@@ -686,7 +778,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
              "    const bool more = tn < NT;\n"
              "    if (more) { __builtin_amdgcn_s_setprio(2); issue(tile_base(tn)); __builtin_amdgcn_s_setprio(0); }\n";
         for (int pi = 0; pi < g.npasses; ++pi) {
-            emit_pass(o, opt, v, v.passes[pi], threads);
+            emit_pass(o, opt, v, v.passes[pi], threads, 0, false, false, pi + 1 == g.npasses);
             o += "    __syncthreads();\n";
         }
         o += "    __builtin_amdgcn_s_setprio(2);\n"
@@ -705,7 +797,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
              "    const bool more = tn < NT;\n"
              "    if (more) issue(tile_base(tn));\n";
         for (int pi = 0; pi < g.npasses; ++pi) {
-            emit_pass(o, opt, v, v.passes[pi], threads);
+            emit_pass(o, opt, v, v.passes[pi], threads, 0, false, false, pi + 1 == g.npasses);
             if (pi + 1 < g.npasses) o += "    __syncthreads();\n";
         }
         o += "    if (more) put(U);\n"

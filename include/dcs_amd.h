@@ -27,7 +27,8 @@
  *   plan                      of scan_step_scaled per step)            dcs_rankwidth/gpu_kernel.py:58-101
  * dcs_plan_kernel_source has no counterpart in the reference: it returns the generated group kernels
  * of a plan without a device, for inspection and for emulating them on the CPU.  Neither has
- * dcs_contract_batch (the lanes of several plans, circuits of different widths, in shared launches).
+ * dcs_contract_batch (the lanes of several plans, circuits of different widths, in shared launches),
+ * nor dcs_plan_factoring (which group kernels, f0-factored or not, the last contraction ran).
  */
 #ifndef DCS_AMD_H
 #define DCS_AMD_H
@@ -123,6 +124,15 @@ typedef struct {
                                    4: as 3, but the last register pass stores its results to HBM
                                    from registers and the group's scale is folded into its first
                                    step (no LDS write-back, loads issued a whole tile ahead) */
+    const int32_t *coef_class;  /* DCS_MODE_COMPILED, complex64 storage with lds_storage 1, exact_rounding
+                                   0 and lds_buffers 1..3 (else ignored; NULL: off): per step, the class
+                                   of the ratio f1 / f0 of its local vector, 0 unknown or f0 possibly zero,
+                                   1 real, 2 imaginary, 3 complex.  The group kernels then run a step of
+                                   class > 0 with (1, f1 / f0) and apply the f0s they left pending at the
+                                   group's last step (fewer FP64 operations; the same results up to FP64
+                                   rounding between the storage roundings).  A contraction whose
+                                   parameters do not fit the classes (f0 == 0, or a ratio of another
+                                   class) runs the group kernels without factoring instead. */
 } dcs_plan_desc;
 
 typedef struct {
@@ -155,6 +165,10 @@ void dcs_plan_destroy(dcs_plan *plan);
  * *seconds (may be NULL) receives the compile time (0 when served from the cache).  dcs_contract in
  * DCS_MODE_COMPILED calls it on first use. */
 int dcs_plan_compile(dcs_plan *plan, double *seconds);
+/* Whether the plan's compiled group kernels are f0-factored (dcs_plan_desc.coef_class) and whether the
+ * last dcs_contract in DCS_MODE_COMPILED ran them (0: its parameters did not fit the classes and the
+ * unfactored kernels ran).  No counterpart in the reference, whose kernels are not factored. */
+int dcs_plan_factoring(dcs_plan *plan, int32_t *module_factored, int32_t *last_call_factored);
 /* Source of the group kernels that dcs_plan_compile would build for this description, and the fused
  * group segments of the compiled schedule, without a device.  src (capacity src_cap, may be NULL)
  * receives the NUL-terminated source, cut to fit; *src_len its full length.  For each group segment i

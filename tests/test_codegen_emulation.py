@@ -18,6 +18,7 @@ import shutil
 import subprocess
 from pathlib import Path
 
+import numpy as np
 import pytest
 
 from dcs_amd import _native
@@ -54,18 +55,21 @@ def w20():
     return BlockPlan(inst['graph'], inst['unitaries'], 28, inst['natural_order'], [], list(range(20, 28)))
 
 
-@pytest.mark.skipif(shutil.which('g++') is None, reason='needs the host C++ compiler')
-@pytest.mark.parametrize('storage,tl,dim,policy,buffers,exact,gridcap', CASES)
-def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, policy, buffers, exact, gridcap):
+def _emulate(w20, tmp_path, storage, tl, dim, policy, buffers, exact, gridcap, classes=None, tol=0.0):
+    """Generate the plan's group kernels, run them in the CPU emulator and return its report lines."""
     t = w20.step_table()
+    extra = {} if classes is None else {'coef_class': classes}
     src, segs = _native.kernel_source(t, w20.inplace_maxslots, storage, w20.terminal_mask(), w20.k,
                                       fuse_max_dim=dim, resident_bits=0, tile_log2=tl, exact_rounding=exact,
-                                      pass_max_dim=3, lds_storage=True, group_policy=policy, lds_buffers=buffers)
+                                      pass_max_dim=3, lds_storage=True, group_policy=policy, lds_buffers=buffers,
+                                      **extra)
     assert segs and len(set(s[0] for s in segs)) == len(segs)
     if buffers >= 3 and storage == 0:
         assert 'global_load_lds' in src        # the asynchronous pipeline is what runs here
     if buffers == 4 and storage == 0:
         assert 'E[tb ^ gb0 ^ ' in src          # and its last pass stores from registers
+    if classes is not None:
+        assert '(f0 factored)' in src and "(group's pending factors)" in src
     (tmp_path / 'kernels.inc').write_text(adapt(src))
     names = sorted({int(m) for m in re.findall(r'\bdcs_group_(\d+)\(St ', src)})
     assert names == list(range(len(names)))
@@ -76,15 +80,43 @@ def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, p
                                                          'out_mask')))
     (tmp_path / 'plan.txt').write_text('\n'.join(rows) + '\n')
     (tmp_path / 'groups.txt').write_text(''.join(f'{g} {a} {c} {th}\n' for g, a, c, th in segs))
+    cmd_extra = []
+    if classes is not None:
+        (tmp_path / 'classes.txt').write_text(' '.join(str(int(c)) for c in classes) + '\n')
+        cmd_extra = [str(tmp_path / 'classes.txt'), repr(tol)]
     exe = tmp_path / 'emu'
     subprocess.run(['g++', '-std=c++20', '-O1', '-pthread', '-w', f'-I{EMU}', f'-I{tmp_path}', str(EMU / 'emu.cpp'),
                     '-o', str(exe)], check=True, timeout=600)
-    r = subprocess.run([str(exe), str(tmp_path / 'plan.txt'), str(tmp_path / 'groups.txt'), str(gridcap)],
+    r = subprocess.run([str(exe), str(tmp_path / 'plan.txt'), str(tmp_path / 'groups.txt'), str(gridcap)] + cmd_extra,
                        capture_output=True, text=True, timeout=900)
     print(r.stdout[-3000:])
     lines = r.stdout.splitlines()
     assert sum(line.startswith('group ') for line in lines) == len(segs)
     assert sum(line.startswith('chain unit ') for line in lines) == len(segs)  # resident_bits=0: groups cover all
+    return r, lines
+
+
+@pytest.mark.skipif(shutil.which('g++') is None, reason='needs the host C++ compiler')
+@pytest.mark.parametrize('storage,tl,dim,policy,buffers,exact,gridcap', CASES)
+def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, policy, buffers, exact, gridcap):
+    r, lines = _emulate(w20, tmp_path, storage, tl, dim, policy, buffers, exact, gridcap)
+    assert r.returncode == 0 and lines[-1] == 'RESULT ok (0 bad)', r.stdout[-2000:]
+
+
+# f0-factored kernels: (tile_log2, fuse_max_dim, group_policy, lds_buffers, gridcap)
+FACTORED = [(12, 8, 1, 3, 1), (12, 8, 2, 1, 2), (11, 6, 0, 2, 3)]
+
+
+@pytest.mark.skipif(shutil.which('g++') is None, reason='needs the host C++ compiler')
+@pytest.mark.parametrize('tl,dim,policy,buffers,gridcap', FACTORED)
+def test_factored_kernels_match_steps_on_cpu(w20, tmp_path, tl, dim, policy, buffers, gridcap):
+    """Complex64 group kernels with f0 factoring (rounded once per register pass) against the per-step
+    formulas, within 2e-6 of the largest magnitude, on every group and the whole chain.  Classes are
+    drawn per step (seeded) so that every class, and class-0 steps with f0 == 0, occur in the groups; the
+    emulator draws step parameters that fit them.  A factored kernel that dropped the pending factors,
+    used the wrong part of a ratio, or misplaced a lane sign would be off by O(1)."""
+    classes = np.random.default_rng(5).integers(0, 4, len(w20.inplace_steps)).astype(np.int32)
+    r, lines = _emulate(w20, tmp_path, 0, tl, dim, policy, buffers, False, gridcap, classes=classes, tol=2e-6)
     assert r.returncode == 0 and lines[-1] == 'RESULT ok (0 bad)', r.stdout[-2000:]
 
 

@@ -1,0 +1,82 @@
+"""f0-factored group kernels on the device, through the C ABI.
+
+The compiled complex64 schedule runs each step whose f0 cannot be zero with (1, f1 / f0) and applies the
+pending f0s at each group's last step (BlockPlan.coefficient_classes).  Checks: the factored blocks stay
+within 1e-5 of the complex128 oracle and no further from it than twice the unfactored kernels' error;
+the plan reports that the factored kernels ran; and a contraction whose parameters do not fit the
+classes it was compiled for (classes forced wrong) runs the unfactored kernels, whose blocks equal a
+factor=False engine's bit for bit, with and without graph replay."""
+import numpy as np
+import pytest
+
+from dcs_amd.block_plan import BlockPlan
+from dcs_amd.conditional_block import InplaceConditionalBlockGPU
+from dcs_amd.stabilizer import synthetic_instance
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+@pytest.fixture(scope='module')
+def cases():
+    out = []
+    inst = synthetic_instance(16, 12, 20, seed=0)
+    args = (inst['graph'], inst['unitaries'], 16, inst['natural_order'], [], list(range(16)))
+    out.append(('c1', args, [np.zeros(16, np.uint8)]))
+    inst = synthetic_instance(28, 32, 64, seed=0)
+    args = (inst['graph'], inst['unitaries'], 28, inst['natural_order'], [], list(range(20, 28)))
+    rng = np.random.default_rng(3)
+    out.append(('w20', args, [rng.integers(0, 2, 28).astype(np.uint8) for _ in range(3)]))
+    return out
+
+
+def engine(args, **kw):
+    e = InplaceConditionalBlockGPU(*args)
+    e.alloc(dtype=np.complex64, schedule='compiled', resident_bits=6, **kw)
+    return e
+
+
+def test_factored_blocks_against_oracle(cases):
+    for name, args, xs in cases:
+        bp = BlockPlan(*args)
+        cls = bp.coefficient_classes()
+        assert (cls > 0).sum() > 0.5 * len(cls)      # every T reference vertex factors
+        fac, plain = engine(args), engine(args, factor=False)
+        for i, x in enumerate(xs):
+            print(f'case {name} prefix {i}', flush=True)
+            ref = oracle.block(bp, x)
+            vf, _ = fac.block(x)
+            assert fac.factoring() == (True, True)
+            vp, _ = plain.block(x)
+            assert plain.factoring() == (False, False)
+            ef, ep = rel(vf, ref), rel(vp, ref)
+            print(f'  error vs FP64: factored {ef:.3e} unfactored {ep:.3e}', flush=True)
+            assert ef < 1e-5 and ef <= 2 * ep + 1e-12, (ef, ep)
+        fac.close()
+        plain.close()
+
+
+@pytest.mark.parametrize('graphs', [False, True])
+def test_misfit_classes_run_unfactored(cases, graphs):
+    name, args, xs = cases[1]
+    bp = BlockPlan(*args)
+    cls = bp.coefficient_classes()
+    wrong = np.where(cls == 2, 1, np.where(cls == 1, 2, cls)).astype(np.int32)   # real <-> imaginary
+    print(f'case {name} misfit classes graphs={graphs}', flush=True)
+    mis = engine(args, factor=wrong, graphs=graphs)
+    plain = engine(args, factor=False, graphs=graphs)
+    good = engine(args, graphs=graphs)
+    for x in xs[:2]:
+        vm, _ = mis.block(x)
+        assert mis.factoring() == (True, False)
+        vp, _ = plain.block(x)
+        assert np.array_equal(np.asarray(vm), np.asarray(vp))
+        vg, _ = good.block(x)
+        assert good.factoring() == (True, True)
+        assert rel(vg, vp) < 1e-5
+    for e in (mis, plain, good):
+        e.close()
