@@ -438,7 +438,8 @@ def main():
         rec = json.loads(tf.read_text())
         # counter readings belong to one schedule: use them only for the same launches and bytes
         if (rec.get('launches_per_sample') == float(launches[2]) and rec.get('lanes_per_launch', 1) == fast['lanes']
-                and rec.get('algorithmic_bytes_per_sample') == coset_bytes):
+                and rec.get('algorithmic_bytes_per_sample') == coset_bytes
+                and bool(rec.get('f0_factored', False)) == bool(factored)):
             traffic = rec.get('hbm_bytes_per_sample')
             valu = rec.get('valu_instructions_per_sample')
     # Three ceilings of the group kernels, each over their own time per sample: HBM bytes (algorithmic),

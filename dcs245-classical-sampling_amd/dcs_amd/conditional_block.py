@@ -277,8 +277,8 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    a group allows it (complex64 tiles in LDS, at least two lanes), else as 1; the same
                    arithmetic as 1, so the same blocks bit for bit; 4 as 3 with the last register pass
                    storing to HBM from registers and the scale folded into the group's first step
-        factor     compiled schedule with complex64 tiles rounded once per register pass (lds_buffers
-                   1..3): run every step whose f0 cannot be zero with the coefficients (1, f1 / f0) and
+        factor     compiled schedule with complex64 tiles rounded once per register pass: run every
+                   step whose f0 cannot be zero with the coefficients (1, f1 / f0) and
                    apply the pending f0s at each group's last step (BlockPlan.coefficient_classes says
                    whether f1 / f0 is real, imaginary or complex, which sets the FP64 work of the step);
                    a slice whose parameters do not fit runs the unfactored kernels; an array gives the

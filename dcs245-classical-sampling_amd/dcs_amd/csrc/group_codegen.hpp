@@ -45,8 +45,8 @@ struct CodegenOptions {
     // only where factored() holds.
     const int8_t *cls = nullptr;
     // f0 factoring: complex64 storage rounded once per register pass (the FP64 arithmetic between roundings
-    // is free to change), the scale applied at the store (not folded into a step).
-    bool factored() const { return cls && storage == 0 && !exact && lds_storage && lds_buffers != 4; }
+    // is free to change).
+    bool factored() const { return cls && storage == 0 && !exact && lds_storage; }
 };
 
 namespace codegen {
@@ -229,13 +229,18 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
         const int C = (fac && !fin) ? opt.cls[q.step] : 0;
         emitf(o, "    { // step %d kind %d%s\n", q.step, q.kind, C ? " (f0 factored)" : fin ? " (group's pending factors)" : "");
         emitf(o, "      const Dyn &q = dl[%d];\n", q.step);
-        const bool sc = fold && s == 0;
+        // the store-free pipeline folds the group's power-of-two scale into a step's coefficients: the
+        // first step, or with factoring the last (the factored steps carry no f0 to scale)
+        const bool sc = fac ? (fin && direct) : (fold && s == 0);
         if (C == 0) {
-            if (sc) emitf(o, "      const double q_f1r = q.f1r * scale, q_f1i = q.f1i * scale;\n");
-            else if (fin) emitf(o, "      const double q_f1r = q.g1r, q_f1i = q.g1i;\n");
-            else emitf(o, "      const double q_f1r = q.f1r, q_f1i = q.f1i;\n");
-            emitf(o, sc ? "      const cd f0{q.f0r * scale, q.f0i * scale};\n"
-                        : fin ? "      const cd f0{q.g0r, q.g0i};\n" : "      const cd f0{q.f0r, q.f0i};\n");
+            const char *c0 = fin ? "g0" : "f0", *c1 = fin ? "g1" : "f1";
+            if (sc) {
+                emitf(o, "      const double q_f1r = q.%sr * scale, q_f1i = q.%si * scale;\n", c1, c1);
+                emitf(o, "      const cd f0{q.%sr * scale, q.%si * scale};\n", c0, c0);
+            } else {
+                emitf(o, "      const double q_f1r = q.%sr, q_f1i = q.%si;\n", c1, c1);
+                emitf(o, "      const cd f0{q.%sr, q.%si};\n", c0, c0);
+            }
         }
         emitf(o, "      const int sg = (__popcll(tb & 0x%llxull) + q.ov) & 1;\n", (unsigned long long)q.beta);
         auto u = [&](int E) { return par32(TE[E] & q.bvec); };

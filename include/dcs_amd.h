@@ -125,7 +125,7 @@ typedef struct {
                                    from registers and the group's scale is folded into its first
                                    step (no LDS write-back, loads issued a whole tile ahead) */
     const int32_t *coef_class;  /* DCS_MODE_COMPILED, complex64 storage with lds_storage 1, exact_rounding
-                                   0 and lds_buffers 1..3 (else ignored; NULL: off): per step, the class
+                                   0 (else ignored; NULL: off): per step, the class
                                    of the ratio f1 / f0 of its local vector, 0 unknown or f0 possibly zero,
                                    1 real, 2 imaginary, 3 complex.  The group kernels then run a step of
                                    class > 0 with (1, f1 / f0) and apply the f0s they left pending at the
