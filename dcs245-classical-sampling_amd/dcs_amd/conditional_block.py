@@ -248,7 +248,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
               fuse_max_dim=None, resident_bits=12, tile_log2=None, blocks=1, exact_rounding=False,
-              pass_max_dim=3, lds_storage=True, group_policy=1, lds_buffers=3, graphs=False, context=None,
+              pass_max_dim=3, lds_storage=True, group_policy=None, lds_buffers=None, graphs=False, context=None,
               factor=True):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:85-105).
 
@@ -267,16 +267,18 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    of its LDS tile (11, 12; 13 compiled only), span of one register pass (3, 4); the
                    defaults are the fastest measured on the 64-qubit plans: (8, 12, 3), and (9, 13, 3)
                    for complex128 in the compiled schedule when neither of the first two is given
-        group_policy  0: greedy maximal coset groups; 1 (default): the group boundaries with the fewest
-                   HBM bytes (dynamic programming); 2: bytes and register-pass work summed; 3: the
-                   same two costs overlapped, max plus a quarter of the smaller (both calibrated on
-                   complex64 tiles)
+        group_policy  0: greedy maximal coset groups; 1: the group boundaries with the fewest HBM
+                   bytes (dynamic programming); 2: bytes and register-pass work summed; 3: the same two
+                   costs overlapped, max plus a quarter of the smaller (both calibrated on complex64
+                   tiles, the register passes priced lower for f0-factored kernels).  Default: 3 for the
+                   f0-factored complex64 kernels, else 1 (the fastest measured on the 64-qubit plans)
         lds_buffers  compiled schedule: 1 one LDS tile per workgroup, the next tile waiting in
-                   registers; 2 double-buffered tiles; 3 (default) two tiles, the next one loaded
-                   straight from HBM into LDS (global_load_lds) during the current tile's passes, where
-                   a group allows it (complex64 tiles in LDS, at least two lanes), else as 1; the same
-                   arithmetic as 1, so the same blocks bit for bit; 4 as 3 with the last register pass
-                   storing to HBM from registers and the scale folded into the group's first step
+                   registers; 2 double-buffered tiles; 3 two tiles, the next one loaded straight from
+                   HBM into LDS (global_load_lds) during the current tile's passes, where a group allows
+                   it (complex64 tiles in LDS, at least two lanes), else as 1; the same arithmetic as 1,
+                   so the same blocks bit for bit; 4 as 3 with the loads issued a whole tile ahead and
+                   the last register pass storing to HBM from registers (the group's scale folded into
+                   a step).  Default: 4 for the f0-factored complex64 kernels, else 3
         factor     compiled schedule with complex64 tiles rounded once per register pass: run every
                    step whose f0 cannot be zero with the coefficients (1, f1 / f0) and
                    apply the pending f0s at each group's last step (BlockPlan.coefficient_classes says
@@ -303,6 +305,12 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
             fuse_max_dim, tile_log2 = 9, 13
         fuse_max_dim = 8 if fuse_max_dim is None else fuse_max_dim
         tile_log2 = 12 if tile_log2 is None else tile_log2
+        factored = (self.mode == _native.MODE_COMPILED and not self.use_fp64 and self._c64_storage == _native.STORAGE_C64
+                    and lds_storage and not exact_rounding and not (factor is False or factor is None))
+        if group_policy is None:
+            group_policy = 3 if factored else 1
+        if lds_buffers is None:
+            lds_buffers = 4 if factored else 3
         self.lanes = int(lanes)
         self._own_ctx = context is None
         self.ctx = _native.Context(device) if context is None else context

@@ -1093,6 +1093,17 @@ std::vector<int> group_ends_from(const dcs_plan *p, int s, int D) {
     return ends;
 }
 
+// Codegen options of the plan's group kernels; with factored, the f0 factoring of the plan's coefficient
+// classes where the schedule allows it.
+CodegenOptions codegen_options(const dcs_plan *p, bool factored) {
+    CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0, p->lds_buffers};
+    if (factored && !p->coef_class.empty()) opt.cls = p->coef_class.data();
+    return opt;
+}
+
+// Whether the plan's compiled schedule runs f0-factored group kernels by default.
+bool factoring_on(const dcs_plan *p) { return codegen_options(p, true).factored(); }
+
 // Group boundaries over the non-resident runs of the plan.  Policy 0: greedy maximal groups (extend
 // while the span stays <= D).  Policy 1: the partition with the fewest HBM bytes, by dynamic
 // programming over group ends: a group reads its live entries once and writes them once, so its cost
@@ -1136,8 +1147,11 @@ void choose_groups(const dcs_plan *p, int D, std::vector<int> &group_end) {
             // policy 1: HBM entries moved (+ a launch allowance of 2^20 entries); 2: entries at 1.73
             // and element-steps over the group's span at 0.31 (measured costs, summed); 3: the same two
             // times overlapped, max(memory, compute) + a quarter of the smaller one (a group's loads and
-            // stores run during its passes, three quarters of the shorter phase hidden as measured)
-            const double w_mem = p->group_policy >= 2 ? 1.73 : 1.0, w_cmp = p->group_policy >= 2 ? 0.31 : 0.0;
+            // stores run during its passes, three quarters of the shorter phase hidden as measured).
+            // f0-factored kernels issue 0.61 times the VALU instructions per element-step (7.1 against
+            // 11.7 lane instructions, SQ_INSTS_VALU on the 64-qubit plan), so their steps cost 0.19.
+            const double w_mem = p->group_policy >= 2 ? 1.73 : 1.0;
+            const double w_cmp = p->group_policy >= 2 ? (factoring_on(p) ? 0.19 : 0.31) : 0.0;
             const bool overlap = p->group_policy == 3;
             for (int i = s; i < b; ++i) {
                 if (best[i - s] >= inf) continue;
@@ -1525,17 +1539,6 @@ ModuleCache &module_cache() {
     static ModuleCache c;
     return c;
 }
-
-// Codegen options of the plan's group kernels; with factored, the f0 factoring of the plan's coefficient
-// classes where the schedule allows it.
-CodegenOptions codegen_options(const dcs_plan *p, bool factored) {
-    CodegenOptions opt{p->storage, p->tile_log2, p->exact_rounding != 0, p->lds_storage != 0, p->lds_buffers};
-    if (factored && !p->coef_class.empty()) opt.cls = p->coef_class.data();
-    return opt;
-}
-
-// Whether the plan's compiled schedule runs f0-factored group kernels by default.
-bool factoring_on(const dcs_plan *p) { return codegen_options(p, true).factored(); }
 
 // Source of the module holding one generated kernel per fused group of the plan's schedule.
 std::string group_module_source(const dcs_plan *p, const std::vector<GroupDev> &groups, const std::vector<PassDev> &passes,

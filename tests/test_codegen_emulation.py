@@ -104,7 +104,7 @@ def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, p
 
 
 # f0-factored kernels: (tile_log2, fuse_max_dim, group_policy, lds_buffers, gridcap)
-FACTORED = [(12, 8, 1, 3, 1), (12, 8, 2, 4, 3), (12, 8, 2, 1, 2), (11, 6, 0, 2, 3)]
+FACTORED = [(12, 8, 1, 3, 1), (12, 8, 3, 4, 3), (12, 8, 2, 1, 2), (11, 6, 0, 2, 3)]
 
 
 @pytest.mark.skipif(shutil.which('g++') is None, reason='needs the host C++ compiler')

@@ -4,8 +4,8 @@ The compiled complex64 schedule runs each step whose f0 cannot be zero with (1, 
 pending f0s at each group's last step (BlockPlan.coefficient_classes).  Checks: the factored blocks stay
 within 1e-5 of the complex128 oracle and no further from it than twice the unfactored kernels' error;
 the plan reports that the factored kernels ran; and a contraction whose parameters do not fit the
-classes it was compiled for (classes forced wrong) runs the unfactored kernels, whose blocks equal a
-factor=False engine's bit for bit, with and without graph replay."""
+classes it was compiled for (classes forced wrong) runs the unfactored kernels of the same schedule,
+bit for bit, with and without graph replay."""
 import numpy as np
 import pytest
 
@@ -62,21 +62,26 @@ def test_factored_blocks_against_oracle(cases):
 
 @pytest.mark.parametrize('graphs', [False, True])
 def test_misfit_classes_run_unfactored(cases, graphs):
+    """Classes forced wrong (real and imaginary swapped): every contraction falls back to the unfactored
+    kernels of the same schedule, whose blocks equal those of an engine whose classes are all 0 (the
+    same group boundaries, every step with (f0, f1), the last with (1 f0, 1 f1)) bit for bit."""
     name, args, xs = cases[1]
     bp = BlockPlan(*args)
     cls = bp.coefficient_classes()
     wrong = np.where(cls == 2, 1, np.where(cls == 1, 2, cls)).astype(np.int32)   # real <-> imaginary
     print(f'case {name} misfit classes graphs={graphs}', flush=True)
     mis = engine(args, factor=wrong, graphs=graphs)
-    plain = engine(args, factor=False, graphs=graphs)
+    zero = engine(args, factor=np.zeros_like(cls), graphs=graphs)
     good = engine(args, graphs=graphs)
+    assert mis.schedule_options == zero.schedule_options == good.schedule_options
     for x in xs[:2]:
         vm, _ = mis.block(x)
         assert mis.factoring() == (True, False)
-        vp, _ = plain.block(x)
-        assert np.array_equal(np.asarray(vm), np.asarray(vp))
+        vz, _ = zero.block(x)
+        assert zero.factoring() == (True, True)
+        assert np.array_equal(np.asarray(vm), np.asarray(vz))
         vg, _ = good.block(x)
         assert good.factoring() == (True, True)
-        assert rel(vg, vp) < 1e-5
-    for e in (mis, plain, good):
+        assert rel(vg, vz) < 1e-5
+    for e in (mis, zero, good):
         e.close()
