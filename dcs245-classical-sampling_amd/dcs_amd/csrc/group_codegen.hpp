@@ -131,6 +131,10 @@ template <int C, int U> __device__ __forceinline__ cd radd(cd a, cd p, cd b) {
     else if constexpr (C == 2) return cd{fma(-ki, b.i, a.r), fma(ki, b.r, a.i)};
     else return cd{fma(-ki, b.i, fma(kr, b.r, a.r)), fma(ki, b.r, fma(kr, b.i, a.i))};
 }
+// A complex64 result stored non-temporally (a group's output is not read again before the next group
+// runs over the whole frontier; 2% faster on the 64-qubit plan, as is the non-temporal bit of the
+// global-to-LDS loads)
+__device__ __forceinline__ void dcs_store_nt(float2 *p, float2 r) { typedef float v2 __attribute__((ext_vector_type(2))); __builtin_nontemporal_store(v2{r.x, r.y}, (v2 *)p); }
 // NaN-propagating maximum (llvm.maximum: one v_maximum3_f32 on gfx950)
 __device__ __forceinline__ float max_nan_f(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 )";
@@ -442,7 +446,7 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
                 const unsigned long long C = (unsigned long long)addr_of(v, TE[E]);
                 const bool rt = (DO & ~pivm) != 0;
                 if (rt) emitf(o, "    if (dlo%d == %uu) {\n", m, TE[E] & DO & ~pivm);
-                emitf(o, "    { const float2 r = LP(x%d_%d); E[tb ^ gb%d ^ 0x%llxull] = r; "
+                emitf(o, "    { const float2 r = LP(x%d_%d); dcs_store_nt(&E[tb ^ gb%d ^ 0x%llxull], r); "
                          "lmf = max_nan_f(lmf, r.x * r.x + r.y * r.y); }\n", m, E, m, C);
                 if (rt) emitf(o, "    }\n");
             }
@@ -634,7 +638,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
             emitf(o, "    if (!tdout%s) { const float2 a = P[%s]; float2 r;\n", gexpr.c_str(), at.c_str());
             if (fm == 1) o += "      r = make_float2(a.x * sf, a.y * sf);\n";
             else o += "      r = make_float2((float)((double)a.x * scale), (float)((double)a.y * scale));\n";
-            emitf(o, "      E[tbx ^ pth ^ 0x%llxull] = r;\n", C);
+            emitf(o, "      dcs_store_nt(&E[tbx ^ pth ^ 0x%llxull], r);\n", C);
             o += "      lmf = max_nan_f(lmf, r.x * r.x + r.y * r.y); }\n";
         } else {
             emitf(o, "    if (!tdout%s) put_scaled(E, tbx ^ pth ^ 0x%llxull, LG(P[%s]), scale, lm);\n", gexpr.c_str(), C,
@@ -699,7 +703,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
             const unsigned long long Dj = (unsigned long long)(lane_dep_const(g, l) ^ addr_of(v, t));
             if (!(t & g.dead_in_t))
                 emitf(o, "    if (!tdin2) __builtin_amdgcn_global_load_lds((dcs_gptr)(E + ((tbx ^ pa ^ 0x%llxull) & ~1ull)), "
-                         "(dcs_lptr)(P + %uu + 2u * (vt & ~63u)), 16, 0, 0);\n", Dj, x);
+                         "(dcs_lptr)(P + %uu + 2u * (vt & ~63u)), 16, 0, 2);\n", Dj, x);
         }
         o += "  };\n";
         // vmcnt(0) with the other counters at their maximum (gfx9 encoding): every load into LDS landed

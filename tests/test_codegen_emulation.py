@@ -40,12 +40,15 @@ CASES = [
 
 def adapt(src):
     """The generated source as the host compiler takes it: no fixed-width typedefs (the shim's headers
-    have them), no GPU attributes, the LDS-only barrier as a workgroup barrier."""
+    have them), no GPU attributes, the LDS-only barrier as a workgroup barrier, the non-temporal store
+    as a plain store."""
     src = '\n'.join(line for line in src.split('\n') if not re.match(r'typedef .*int.*_t;', line))
     src = src.replace('__attribute__((amdgpu_waves_per_eu(4)))', '')
     src = re.sub(r'__attribute__\(\(address_space\(\d\)\)\)', '', src)
     src, n = re.subn(r'\{ asm volatile\(.*?s_barrier.*?\); \}', '{ __syncthreads(); }', src)
     assert n == 1, 'expected one LDS barrier definition in the prelude'
+    src, n = re.subn(r'void dcs_store_nt\(float2 \*p, float2 r\) \{.*\}', 'void dcs_store_nt(float2 *p, float2 r) { *p = r; }', src)
+    assert n == 1, 'expected one non-temporal store helper in the prelude'
     return src
 
 
