@@ -278,7 +278,9 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    it (complex64 tiles in LDS, at least two lanes), else as 1; the same arithmetic as 1,
                    so the same blocks bit for bit; 4 as 3 with the loads issued a whole tile ahead and
                    the last register pass storing to HBM from registers (the group's scale folded into
-                   a step).  Default: 4 for the f0-factored complex64 kernels, else 3
+                   a step); 5 as 4 with the live inputs of the next two tiles in flight, in a ring of
+                   compact LDS buffers, for groups whose input is a half or a quarter live (the same
+                   blocks as 4 bit for bit).  Default: 4 for the f0-factored complex64 kernels, else 3
         factor     compiled schedule with complex64 tiles rounded once per register pass: run every
                    step whose f0 cannot be zero with the coefficients (1, f1 / f0) and
                    apply the pending f0s at each group's last step (BlockPlan.coefficient_classes says

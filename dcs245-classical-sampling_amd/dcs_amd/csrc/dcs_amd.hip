@@ -2009,7 +2009,7 @@ int check_plan_desc(const dcs_plan_desc *d) {
     if (d->tile_log2 < 11 || d->tile_log2 > 13)
         return fail(DCS_ERR_INVALID, "tile_log2 must be 11 or 12 (13: DCS_MODE_COMPILED only)");
     if (d->group_policy < 0 || d->group_policy > 3) return fail(DCS_ERR_INVALID, "group_policy must be 0..3");
-    if (d->lds_buffers < 0 || d->lds_buffers > 4) return fail(DCS_ERR_INVALID, "lds_buffers must be 0..4");
+    if (d->lds_buffers < 0 || d->lds_buffers > 5) return fail(DCS_ERR_INVALID, "lds_buffers must be 0..5");
     if (d->resident_bits != 0 && (d->resident_bits < 6 || d->resident_bits > 12))
         return fail(DCS_ERR_INVALID, "resident_bits must be 0 or 6..12");
     if (d->fuse_max_dim < 0 || d->fuse_max_dim > kMaxDimCompiled)

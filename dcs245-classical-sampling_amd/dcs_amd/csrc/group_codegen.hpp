@@ -185,8 +185,37 @@ inline u64 lane_dep_const(const GroupDev &g, uint32_t l) {
 // last pass, and its results go from registers straight to HBM (rounded to storage, dead outputs
 // skipped, the maximum |stored|^2 kept) instead of back into the LDS tile; gb<m> / dlo<m> are the
 // per-thread address and liveness values set up by emit_direct_setup.
+// Bits of `t` at the set positions of `mask`, packed to the low end in order (constant t).
+inline uint32_t pext32(uint32_t t, uint32_t mask) {
+    uint32_t r = 0;
+    int j = 0;
+    for (int p = 0; p < 32; ++p)
+        if ((mask >> p) & 1) {
+            if ((t >> p) & 1) r |= 1u << j;
+            ++j;
+        }
+    return r;
+}
+
+// The same as an expression over a run-time value named x.
+inline std::string pext_expr(const char *x, uint32_t mask) {
+    std::string e = "(0u";
+    int j = 0;
+    for (int p = 0; p < 32; ++p)
+        if ((mask >> p) & 1) {
+            char buf[96];
+            snprintf(buf, sizeof(buf), " | (((%s >> %d) & 1u) << %d)", x, p, j);
+            e += buf;
+            ++j;
+        }
+    return e + ")";
+}
+
+// csrc (the ring pipeline's first pass): the pass reads its inputs from the compact buffer C, which
+// holds only the entries live before the group (coset coordinates t with t & dead_in_t == 0, at index
+// (pext(t, live) W + l) ^ parity(t & m0)), takes dead inputs as zero, and writes every entry of T.
 inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView &v, const PassDev &ps, int threads,
-                      uint32_t m0 = 0, bool fold = false, bool direct = false, bool last = false) {
+                      uint32_t m0 = 0, bool fold = false, bool direct = false, bool last = false, bool csrc = false) {
     const GroupDev &g = *v.g;
     const int J = ps.nmask, NE = 1 << J, WB = g.wbits, D = g.d;
     const uint32_t norb = (1u << (D - J)) << WB;
@@ -220,7 +249,26 @@ inline void emit_pass(std::string &o, const CodegenOptions &opt, const GroupView
         for (int i = 0; i < J; ++i) emitf(o, "    t%d = insert0(t%d, %d);\n", m, m, ps.piv[i]);
         if (m0) emitf(o, "    const uint32_t b%d = ((t%d << %d) | l%d) ^ (__popc(t%d & %uu) & 1u);\n", m, m, WB, m, m, m0);
         else emitf(o, "    const uint32_t b%d = (t%d << %d) | l%d;\n", m, m, WB, m);
-        for (int E = 0; E < NE; ++E) emitf(o, "    cd x%d_%d = LG(T[b%d ^ %uu]);\n", m, E, m, (TE[E] << WB) ^ (uint32_t)par32(TE[E] & m0));
+        if (!csrc) {
+            for (int E = 0; E < NE; ++E)
+                emitf(o, "    cd x%d_%d = LG(T[b%d ^ %uu]);\n", m, E, m, (TE[E] << WB) ^ (uint32_t)par32(TE[E] & m0));
+            continue;
+        }
+        uint32_t pivm = 0;
+        for (int i = 0; i < J; ++i) pivm |= 1u << ps.piv[i];
+        const uint32_t DI = v.g->dead_in_t, LM = ((1u << D) - 1) & ~DI, DM = DI & ~pivm;
+        char tn[16];
+        snprintf(tn, sizeof(tn), "t%d", m);
+        emitf(o, "    const uint32_t bc%d = ((%s << %d) | l%d) ^ (__popc(t%d & %uu) & 1u);\n", m, pext_expr(tn, LM).c_str(), WB,
+              m, m, m0);
+        if (DM) emitf(o, "    const uint32_t dc%d = t%d & %uu;\n", m, m, DM);
+        for (int E = 0; E < NE; ++E) {
+            const uint32_t te = TE[E];
+            const uint32_t ce = (pext32(te, LM) << WB) ^ (uint32_t)par32(te & m0);
+            if (te & DI & pivm) emitf(o, "    cd x%d_%d = cd{0.0, 0.0};\n", m, E);          // dead for every orbit
+            else if (DM) emitf(o, "    cd x%d_%d = (dc%d == %uu) ? LG(C[bc%d ^ %uu]) : cd{0.0, 0.0};\n", m, E, m, te & DM, m, ce);
+            else emitf(o, "    cd x%d_%d = LG(C[bc%d ^ %uu]);\n", m, E, m, ce);
+        }
     }
     for (int s = 0; s < ps.nsteps; ++s) {
         const PStepDev &q = v.psteps[ps.first_pstep + s];
@@ -522,17 +570,45 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
     // even, swapped when the coset part of a carries bit 0 (the tile base never does, bit 0 being a lane
     // bit).  The LDS index permutation with m0 = the coordinates carrying bit 0 stores both
     // orientations where the 16-byte load puts them.
-    const bool dma = (opt.lds_buffers == 3 || opt.lds_buffers == 4) && c64 && lds_is_storage && g.lane_identity &&
-                     WB >= 1 && !guard && PER >= 2;
+    const bool dma = opt.lds_buffers >= 3 && c64 && lds_is_storage && g.lane_identity && WB >= 1 && !guard && PER >= 2;
     // lds_buffers 4: the asynchronous-load pipeline whose last register pass stores to HBM from registers
-    const bool direct = dma && opt.lds_buffers == 4;
+    // (5: the same where a group does not take the ring pipeline below)
+    const bool direct = dma && opt.lds_buffers >= 4;
+    // lds_buffers 5, ring pipeline: for a group whose input is a half or a quarter live (one or two coset
+    // coordinates born inside it), the live inputs of the next two tiles are in flight at once, in a ring
+    // of three compact LDS buffers beside the working tile (80 KiB or less, two workgroups per CU).  Each
+    // thread issues the same loads and stores every tile (no runtime liveness in the first and last
+    // passes), so the wait for a tile's loads counts exactly the memory operations issued after them.
+    const PassDev &lastp = v.passes[g.npasses - 1];
+    uint32_t last_pivm = 0;
+    for (int i = 0; i < lastp.nmask; ++i) last_pivm |= 1u << lastp.piv[i];
+    const int Dn = __builtin_popcount(g.dead_in_t);
+    const uint32_t SC = Dn <= D ? ((1u << (D - Dn)) << WB) : 0;  // compact entries per tile
+    // per wave and tile: LC loads of compact pairs, ST stores of the last pass (its statically live outputs)
+    const uint32_t LC = SC / 2 / (uint32_t)threads;
+    int ST = 0;
+    {
+        const int J = lastp.nmask, NE = 1 << J;
+        const uint32_t norb_last = (1u << (D - J)) << WB;
+        const int OPT = norb_last >= (uint32_t)threads ? (int)(norb_last / (uint32_t)threads) : 0;
+        for (int E = 0; E < NE; ++E) {
+            uint32_t t = 0;
+            for (int l = 0; l < J; ++l)
+                if ((E >> l) & 1) t ^= lastp.mw[l];
+            if (!(t & g.dead_out_t & last_pivm)) ST += OPT;
+        }
+    }
+    const bool ring = direct && opt.lds_buffers == 5 && Dn >= 1 && Dn <= 2 && (SC / 2) % (uint32_t)threads == 0 &&
+                      (g.dead_out_t & ~last_pivm) == 0 &&
+                      (((1u << (D - lastp.nmask)) << WB) >= (uint32_t)threads) &&
+                      (N + 3 * SC) * 8 <= 80 * 1024 && 2 * (int)LC + 2 * ST <= 63;
     uint32_t m0 = 0;
     if (dma)
         for (int i = 0; i < D; ++i)
             if (addr_of(v, 1u << i) & 1ull) m0 |= 1u << i;
     auto lidx = [&](uint32_t e) { return e ^ (uint32_t)par32((e >> WB) & m0); };
     const bool single = opt.lds_buffers == 1 || (opt.lds_buffers >= 3 && !dma);
-    emitf(o, "  __shared__ Lt buf[%u];\n", single ? N : 2 * N);
+    emitf(o, "  __shared__ Lt buf[%u];\n", ring ? N + 3 * SC : single ? N : 2 * N);
     emitf(o, "  const u64 NT = %lluull;\n", (unsigned long long)g.ntiles);
     o += "  u64 ti = blockIdx.x;\n  if (ti >= NT) return;\n"
          "  const uint32_t tid = threadIdx.x;\n"
@@ -710,6 +786,82 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         o += "  auto dma_wait = [&]() { __builtin_amdgcn_s_waitcnt(0x0F70); };\n";
     }
     // pipeline
+    if (ring) {
+        // Each tile k (the workgroup's k-th): issue the loads of tile k + 2 into ring slot (c + 2) % 3 (read
+        // by tile k - 1's first pass, which every wave has finished: a barrier follows each pass), wait
+        // for tile k's loads, synchronise, run the first pass from ring slot c into T, then the other
+        // passes on T, the last one storing to HBM from registers.  Per wave and tile: LC loads, ST
+        // stores, all unconditional; memory operations complete in issue order, so tile k's loads have
+        // landed once at most LC (tile k + 1) + LC (tile k + 2) + ST (tile k - 1) + ST (tile k - 2) of
+        // the operations issued after them are outstanding (fewer near the start and the end).
+        const uint32_t LM = ((1u << D) - 1) & ~g.dead_in_t;
+        // compact pair x (even) of tile base tbx: entries x, x + 1 at t = pdep(x >> WB, LM), l = x & (W - 1)
+        auto ccontrib = [&](int p) -> u64 {  // address part of bit p of a compact entry index
+            if (p < WB) return lane_dep_const(g, 1u << p);
+            int j = p - WB, q = 0;
+            for (int b = 0; b < D; ++b)
+                if ((LM >> b) & 1) {
+                    if (q == j) return addr_of(v, 1u << b);
+                    ++q;
+                }
+            return 0;
+        };
+        std::string pc = "0ull";
+        for (int p = 1; p <= EB; ++p) {
+            char b[160];
+            snprintf(b, sizeof(b), " ^ ((0ull - (u64)((vt >> %d) & 1u)) & 0x%llxull)", p - 1, (unsigned long long)ccontrib(p));
+            pc += b;
+        }
+        emitf(o, "  const u64 pc = %s;\n", pc.c_str());
+        o += "  auto ring_fill = [&](Lt *__restrict__ P, u64 tbx) {\n";
+        for (uint32_t k = 0; k < LC; ++k) {
+            const uint32_t x = 2u * k * span;
+            u64 Dk = 0;
+            for (int p = 0; p < 32; ++p)
+                if ((x >> p) & 1) Dk ^= ccontrib(p);
+            emitf(o, "    __builtin_amdgcn_global_load_lds((dcs_gptr)(E + ((tbx ^ pc ^ 0x%llxull) & ~1ull)), "
+                     "(dcs_lptr)(P + %uu + 2u * (vt & ~63u)), 16, 0, 2);\n", (unsigned long long)Dk, x);
+        }
+        o += "  };\n";
+        // s_waitcnt vmcnt(n), the other counters at their maximum (gfx9 encoding: vmcnt bits 3:0 and 15:14)
+        auto wait_imm = [](int n) { return (n & 15) | (((n >> 4) & 3) << 14) | 0x0F70; };
+        emit_direct_setup(o, v, lastp, threads);
+        emitf(o, "  const u64 G = gridDim.x;\n");
+        o += "  u64 tb = tile_base(ti);\n";
+        o += "  auto passes = [&](Lt *__restrict__ T, Lt *__restrict__ C, Lt *__restrict__ F, bool more2, u64 tn2, int k, bool more) {\n"
+             "    if (more2) { __builtin_amdgcn_s_setprio(2); ring_fill(F, tile_base(tn2)); __builtin_amdgcn_s_setprio(0); }\n";
+        // the wait: k in {0, 1, >= 2} times (more, more2) in {(1, 1), (1, 0), (0, 0)}
+        o += "    {\n      const int ks = k < 2 ? k : 2;\n";
+        for (int ks = 0; ks <= 2; ++ks)
+            for (int mm = 0; mm <= 2; ++mm) {  // mm = more + more2
+                const int n = (int)LC * mm + ST * ks;
+                emitf(o, "      %sif (ks == %d && (int)more + (int)more2 == %d) __builtin_amdgcn_s_waitcnt(0x%x);\n",
+                      (ks || mm) ? "else " : "", ks, mm, wait_imm(n));
+            }
+        o += "    }\n    dcs_lds_barrier();\n";
+        for (int pi = 0; pi < g.npasses; ++pi) {
+            emit_pass(o, opt, v, v.passes[pi], threads, m0, pi == 0, pi + 1 == g.npasses, pi + 1 == g.npasses, pi == 0);
+            if (pi + 1 < g.npasses) o += "    dcs_lds_barrier();\n";
+        }
+        o += "  };\n";
+        emitf(o, "  Lt *const R0 = buf + %u, *const R1 = buf + %u, *const R2 = buf + %u;\n", N, N + SC, N + 2 * SC);
+        o += "  ring_fill(R0, tb);\n"
+             "  if (ti + G < NT) ring_fill(R1, tile_base(ti + G));\n"
+             "  int k = 0;\n"
+             "  for (;;) {\n"
+             "    const u64 tn = ti + G, tn2 = ti + 2 * G;\n"
+             "    const bool more = tn < NT, more2 = tn2 < NT;\n"
+             "    const int c = k % 3;\n"
+             "    Lt *const C = c == 0 ? R0 : c == 1 ? R1 : R2;\n"
+             "    Lt *const F = c == 0 ? R2 : c == 1 ? R0 : R1;\n"
+             "    passes(buf, C, F, more2, tn2, k, more);\n"
+             "    if (!more) break;\n"
+             "    ++k;\n    ti = tn;\n    tb = tile_base(ti);\n"
+             "  }\n";
+        o += "  commit_max((double)lmf, &dmax[(u64)li * nunits + unit]);\n";
+        o += "}\n";
+        return o;
+    }
     if (direct) {
         // Each tile: issue the next tile's loads into U (free: the previous tile's last pass stored its
         // results from registers), run the register passes on T, the last one storing to HBM; then wait

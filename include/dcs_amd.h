@@ -123,7 +123,10 @@ typedef struct {
                                    complex64 tiles allow it (else as 1); same results as 1;
                                    4: as 3, but the last register pass stores its results to HBM
                                    from registers and the group's scale is folded into its first
-                                   step (no LDS write-back, loads issued a whole tile ahead) */
+                                   step (no LDS write-back, loads issued a whole tile ahead);
+                                   5: as 4, and where a group's input is a half or a quarter live, a
+                                   ring of three compact LDS buffers holds the live inputs of the next
+                                   two tiles in flight (same results as 4) */
     const int32_t *coef_class;  /* DCS_MODE_COMPILED, complex64 storage with lds_storage 1, exact_rounding
                                    0 (else ignored; NULL: off): per step, the class
                                    of the ratio f1 / f0 of its local vector, 0 unknown or f0 possibly zero,

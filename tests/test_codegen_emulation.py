@@ -31,6 +31,7 @@ EMU = Path(__file__).resolve().parent / 'emu'
 CASES = [
     (0, 12, 8, 1, 3, True, 1),    # complex64, asynchronous loads, one workgroup walks every tile
     (0, 12, 8, 1, 4, True, 1),    # complex64, asynchronous loads, last pass stores from registers
+    (0, 12, 8, 3, 5, True, 2),    # complex64, ring of compact input buffers two tiles ahead where a group allows
     (0, 12, 8, 2, 4, True, 3),    # the same with three workgroups and bytes-and-passes groups
     (0, 11, 6, 0, 2, True, 3),    # complex64, two register-fed LDS tiles on 2^11 tiles, greedy groups
     (1, 12, 8, 1, 1, False, 2),   # complex128, one LDS tile
@@ -69,8 +70,10 @@ def _emulate(w20, tmp_path, storage, tl, dim, policy, buffers, exact, gridcap, c
     assert segs and len(set(s[0] for s in segs)) == len(segs)
     if buffers >= 3 and storage == 0:
         assert 'global_load_lds' in src        # the asynchronous pipeline is what runs here
-    if buffers == 4 and storage == 0:
+    if buffers >= 4 and storage == 0:
         assert 'E[tb ^ gb0 ^ ' in src          # and its last pass stores from registers
+    if buffers == 5 and storage == 0:
+        assert 'ring_fill(F, ' in src          # some group takes the ring of compact input buffers
     if classes is not None:
         assert '(f0 factored)' in src and "(group's pending factors)" in src
     (tmp_path / 'kernels.inc').write_text(adapt(src))
@@ -107,7 +110,7 @@ def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, p
 
 
 # f0-factored kernels: (tile_log2, fuse_max_dim, group_policy, lds_buffers, gridcap)
-FACTORED = [(12, 8, 1, 3, 1), (12, 8, 3, 4, 3), (12, 8, 2, 1, 2), (11, 6, 0, 2, 3)]
+FACTORED = [(12, 8, 1, 3, 1), (12, 8, 3, 4, 3), (12, 8, 0, 5, 1), (12, 8, 2, 1, 2), (11, 6, 0, 2, 3)]
 
 
 @pytest.mark.skipif(shutil.which('g++') is None, reason='needs the host C++ compiler')

@@ -85,3 +85,32 @@ def test_misfit_classes_run_unfactored(cases, graphs):
         assert rel(vg, vz) < 1e-5
     for e in (mis, zero, good):
         e.close()
+
+
+@pytest.mark.parametrize('factor', [False, True])
+def test_ring_pipeline_equals_direct_store(factor):
+    """The ring of compact input buffers (lds_buffers=5: the live inputs of the next two tiles in flight,
+    waits counted per wave) runs the same arithmetic as the direct-store pipeline (lds_buffers=4): on
+    the 32q/d40/120T ring plan (width 29, 4 GiB complex64 frontier, six of its groups take the ring)
+    every block equals it bit for bit, per-step rounding and f0-factored, over four protocol prefixes
+    two lanes at a time."""
+    from dcs_amd.sampling_protocol import make_protocol
+    from dcs_amd.width_planner import best_rotation
+    inst = synthetic_instance(32, 40, 120, seed=0)
+    order, opn, width = best_rotation(inst['graph'], inst['unitaries'], 32, inst['natural_order'], 16, stride=2)
+    args = (inst['graph'], inst['unitaries'], 32, order, [], opn)
+    xs = [np.array([int(c) for c in r['input_data_bits']], np.uint8) for r in make_protocol(2, 4, 32, opn, 1.0)['records']]
+    kw = dict(dtype=np.complex64, schedule='compiled', lanes=2, blocks=2, group_policy=1, exact_rounding=not factor,
+              factor=factor)
+    out = {}
+    for buffers in (4, 5):
+        print(f'case 32q ring plan lds_buffers={buffers} factor={factor}', flush=True)
+        e = InplaceConditionalBlockGPU(*args)
+        e.alloc(lds_buffers=buffers, **kw)
+        out[buffers] = []
+        for i in (0, 2):
+            vecs, _ = e.block_many(xs[i:i + 2])
+            out[buffers] += [np.asarray(v) for v in vecs]
+        e.close()
+    for a, b in zip(out[4], out[5]):
+        assert np.array_equal(a, b)
