@@ -281,8 +281,8 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    a step); 5 as 4 with the live inputs of the next two tiles in flight, in a ring of
                    compact LDS buffers, for groups whose input is a half or a quarter live (the same
                    blocks as 4 bit for bit).  Default: 4 for the f0-factored complex64 kernels, else 3
-        factor     compiled schedule with complex64 tiles rounded once per register pass: run every
-                   step whose f0 cannot be zero with the coefficients (1, f1 / f0) and
+        factor     compiled schedule without per-step rounding (complex128, or tiles rounded once per
+                   register pass): run every step whose f0 cannot be zero with the coefficients (1, f1 / f0) and
                    apply the pending f0s at each group's last step (BlockPlan.coefficient_classes says
                    whether f1 / f0 is real, imaginary or complex, which sets the FP64 work of the step);
                    a slice whose parameters do not fit runs the unfactored kernels; an array gives the

@@ -44,9 +44,10 @@ struct CodegenOptions {
     // 1, 2, 3 the ratio f1 / f0 is real, imaginary or complex and the step runs with (1, f1 / f0).  Used
     // only where factored() holds.
     const int8_t *cls = nullptr;
-    // f0 factoring: complex64 storage rounded once per register pass (the FP64 arithmetic between roundings
-    // is free to change).
-    bool factored() const { return cls && storage == 0 && !exact && lds_storage; }
+    // f0 factoring: wherever the FP64 arithmetic between storage roundings is free to change, i.e. not
+    // the per-step rounding mode; complex64 and binary16 storage also need tiles held in storage
+    // precision in LDS (rounded once per register pass), complex128 holds FP64 tiles either way.
+    bool factored() const { return cls && !exact && (storage == 1 || lds_storage); }
 };
 
 namespace codegen {

@@ -127,8 +127,8 @@ typedef struct {
                                    5: as 4, and where a group's input is a half or a quarter live, a
                                    ring of three compact LDS buffers holds the live inputs of the next
                                    two tiles in flight (same results as 4) */
-    const int32_t *coef_class;  /* DCS_MODE_COMPILED, complex64 storage with lds_storage 1, exact_rounding
-                                   0 (else ignored; NULL: off): per step, the class
+    const int32_t *coef_class;  /* DCS_MODE_COMPILED with exact_rounding 0, and lds_storage 1 unless the
+                                   storage is complex128 (else ignored; NULL: off): per step, the class
                                    of the ratio f1 / f0 of its local vector, 0 unknown or f0 possibly zero,
                                    1 real, 2 imaginary, 3 complex.  The group kernels then run a step of
                                    class > 0 with (1, f1 / f0) and apply the f0s they left pending at the

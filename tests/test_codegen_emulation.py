@@ -109,20 +109,22 @@ def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, p
     assert r.returncode == 0 and lines[-1] == 'RESULT ok (0 bad)', r.stdout[-2000:]
 
 
-# f0-factored kernels: (tile_log2, fuse_max_dim, group_policy, lds_buffers, gridcap)
-FACTORED = [(12, 8, 1, 3, 1), (12, 8, 3, 4, 3), (12, 8, 0, 5, 1), (12, 8, 2, 1, 2), (11, 6, 0, 2, 3)]
+# f0-factored kernels: (storage, tile_log2, fuse_max_dim, group_policy, lds_buffers, gridcap, tolerance)
+FACTORED = [(0, 12, 8, 1, 3, 1, 2e-6), (0, 12, 8, 3, 4, 3, 2e-6), (0, 12, 8, 0, 5, 1, 2e-6), (0, 12, 8, 2, 1, 2, 2e-6),
+            (0, 11, 6, 0, 2, 3, 2e-6), (1, 12, 8, 1, 1, 2, 1e-12), (2, 12, 8, 2, 3, 2, 2e-2)]
 
 
 @pytest.mark.skipif(shutil.which('g++') is None, reason='needs the host C++ compiler')
-@pytest.mark.parametrize('tl,dim,policy,buffers,gridcap', FACTORED)
-def test_factored_kernels_match_steps_on_cpu(w20, tmp_path, tl, dim, policy, buffers, gridcap):
-    """Complex64 group kernels with f0 factoring (rounded once per register pass) against the per-step
-    formulas, within 2e-6 of the largest magnitude, on every group and the whole chain.  Classes are
-    drawn per step (seeded) so that every class, and class-0 steps with f0 == 0, occur in the groups; the
-    emulator draws step parameters that fit them.  A factored kernel that dropped the pending factors,
-    used the wrong part of a ratio, or misplaced a lane sign would be off by O(1)."""
+@pytest.mark.parametrize('storage,tl,dim,policy,buffers,gridcap,tol', FACTORED)
+def test_factored_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, policy, buffers, gridcap, tol):
+    """Group kernels with f0 factoring against the per-step formulas, within tol of the largest magnitude
+    (complex64 and binary16 round once per register pass instead of once per step: 2e-6 and 2e-2;
+    complex128 never rounds: 1e-12), on every group and the whole chain.  Classes are drawn per step
+    (seeded) so that every class, and class-0 steps with f0 == 0, occur in the groups; the emulator
+    draws step parameters that fit them.  A factored kernel that dropped the pending factors, used the
+    wrong part of a ratio, or misplaced a lane sign would be off by O(1)."""
     classes = np.random.default_rng(5).integers(0, 4, len(w20.inplace_steps)).astype(np.int32)
-    r, lines = _emulate(w20, tmp_path, 0, tl, dim, policy, buffers, False, gridcap, classes=classes, tol=2e-6)
+    r, lines = _emulate(w20, tmp_path, storage, tl, dim, policy, buffers, False, gridcap, classes=classes, tol=tol)
     assert r.returncode == 0 and lines[-1] == 'RESULT ok (0 bad)', r.stdout[-2000:]
 
 
