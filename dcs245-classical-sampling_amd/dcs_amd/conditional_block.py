@@ -274,13 +274,14 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    f0-factored complex64 kernels, else 1 (the fastest measured on the 64-qubit plans)
         lds_buffers  compiled schedule: 1 one LDS tile per workgroup, the next tile waiting in
                    registers; 2 double-buffered tiles; 3 two tiles, the next one loaded straight from
-                   HBM into LDS (global_load_lds) during the current tile's passes, where a group allows
-                   it (complex64 tiles in LDS, at least two lanes), else as 1; the same arithmetic as 1,
-                   so the same blocks bit for bit; 4 as 3 with the loads issued a whole tile ahead and
+                   HBM into LDS (global_load_lds) during the current tile's passes and the finished
+                   tile stored in 16-byte pairs, where a group allows it (complex64 tiles in LDS, at
+                   least two lanes), else as 1; the same arithmetic as 1, so the same blocks bit for
+                   bit; 4 as 3 with the loads issued a whole tile ahead and
                    the last register pass storing to HBM from registers (the group's scale folded into
                    a step); 5 as 4 with the live inputs of the next two tiles in flight, in a ring of
                    compact LDS buffers, for groups whose input is a half or a quarter live (the same
-                   blocks as 4 bit for bit).  Default: 4 for the f0-factored complex64 kernels, else 3
+                   blocks as 4 bit for bit).  Default: 3 (the fastest measured on the 64-qubit plans)
         factor     compiled schedule without per-step rounding (complex128, or tiles rounded once per
                    register pass): run every step whose f0 cannot be zero with the coefficients (1, f1 / f0) and
                    apply the pending f0s at each group's last step (BlockPlan.coefficient_classes says
@@ -312,7 +313,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
         if group_policy is None:
             group_policy = 3 if factored else 1
         if lds_buffers is None:
-            lds_buffers = 4 if factored else 3
+            lds_buffers = 3
         self.lanes = int(lanes)
         self._own_ctx = context is None
         self.ctx = _native.Context(device) if context is None else context

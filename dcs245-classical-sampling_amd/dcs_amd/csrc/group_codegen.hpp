@@ -136,6 +136,8 @@ template <int C, int U> __device__ __forceinline__ cd radd(cd a, cd p, cd b) {
 // runs over the whole frontier; 2% faster on the 64-qubit plan, as is the non-temporal bit of the
 // global-to-LDS loads)
 __device__ __forceinline__ void dcs_store_nt(float2 *p, float2 r) { typedef float v2 __attribute__((ext_vector_type(2))); __builtin_nontemporal_store(v2{r.x, r.y}, (v2 *)p); }
+// Two neighbouring complex64 results in one 16-byte non-temporal store (p 16-byte aligned)
+__device__ __forceinline__ void dcs_store_nt2(float2 *p, float2 a, float2 b) { typedef float v4 __attribute__((ext_vector_type(4))); __builtin_nontemporal_store(v4{a.x, a.y, b.x, b.y}, (v4 *)p); }
 // NaN-propagating maximum (llvm.maximum: one v_maximum3_f32 on gfx950)
 __device__ __forceinline__ float max_nan_f(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 )";
@@ -785,6 +787,33 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         o += "  };\n";
         // vmcnt(0) with the other counters at their maximum (gfx9 encoding): every load into LDS landed
         o += "  auto dma_wait = [&]() { __builtin_amdgcn_s_waitcnt(0x0F70); };\n";
+        if (!direct) {
+            // The stores of the asynchronous-load pipeline mirror its loads: each lane stores the 16 bytes
+            // of LDS entries x, x + 1, which hold the aligned pair of addresses in memory order, to that
+            // pair (scaled, with their maxima), skipping pairs dead at the group's end.  One 16-byte store
+            // per two entries instead of two 8-byte ones: 7% faster than those on the 64-qubit plan.
+            emitf(o, "  const bool tdout2 = ((((uint32_t)vt << 1) >> %d) & %uu) != 0;\n", WB, g.dead_out_t);
+            o += "  auto store_pairs = [&](Lt *P, u64 tbx) {\n   if (sfast) {\n";
+            for (int fm = 1; fm <= 2; ++fm) {
+                if (fm == 2) o += "   } else {\n";
+                for (int j = 0; j < PER / 2; ++j) {
+                    const uint32_t x = 2u * (uint32_t)j * span;
+                    const uint32_t t = x >> WB, l = x & (W - 1);
+                    if (t & g.dead_out_t) continue;
+                    const unsigned long long Dj = (unsigned long long)(lane_dep_const(g, l) ^ addr_of(v, t));
+                    emitf(o, "    if (!tdout2) { const float2 a = P[%uu + 2u * vt], b = P[%uu + 2u * vt + 1u]; float2 ra, rb;\n",
+                          x, x);
+                    if (fm == 1)
+                        o += "      ra = make_float2(a.x * sf, a.y * sf); rb = make_float2(b.x * sf, b.y * sf);\n";
+                    else
+                        o += "      ra = make_float2((float)((double)a.x * scale), (float)((double)a.y * scale));\n"
+                             "      rb = make_float2((float)((double)b.x * scale), (float)((double)b.y * scale));\n";
+                    emitf(o, "      dcs_store_nt2(E + ((tbx ^ pa ^ 0x%llxull) & ~1ull), ra, rb);\n", Dj);
+                    o += "      lmf = max_nan_f(lmf, max_nan_f(ra.x * ra.x + ra.y * ra.y, rb.x * rb.x + rb.y * rb.y)); }\n";
+                }
+            }
+            o += "   }\n  };\n";
+        }
     }
     // pipeline
     if (ring) {
@@ -896,9 +925,9 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
     if (dma) {
         // the tile body takes the two tiles as restrict pointers: the loads into U and the passes'
         // accesses to T get distinct alias scopes, so no wait for the loads is placed before the passes.
-        // U holds the previous tile's results until they are stored: each wave stores them after its
-        // part of the first register pass (mixing the stores with other waves' FP64 work), and the next
-        // tile's loads into U follow the barrier that ends that pass.
+        // U holds the previous tile's results until they are stored: each wave stores them (16-byte
+        // pairs) after its part of the first register pass (mixing the stores with other waves' FP64
+        // work), and the next tile's loads into U follow the barrier that ends that pass.
         o += "  u64 tb = tile_base(ti);\n";
         o += "  auto passes = [&](Lt *__restrict__ T, Lt *__restrict__ U, bool more, u64 tn, bool prev, u64 tprev) {\n"
              "    const u64 tbn = more ? tile_base(tn) : 0ull;\n"
@@ -906,7 +935,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
         for (int pi = 0; pi < g.npasses; ++pi) {
             emit_pass(o, opt, v, v.passes[pi], threads, m0, false, false, pi + 1 == g.npasses);
             if (pi == 0) {
-                o += "    if (prev) { __builtin_amdgcn_s_setprio(2); store(U, tprev); __builtin_amdgcn_s_setprio(0); }\n"
+                o += "    if (prev) { __builtin_amdgcn_s_setprio(2); store_pairs(U, tprev); __builtin_amdgcn_s_setprio(0); }\n"
                      "    dcs_lds_barrier();\n"
                      "    if (more && prev) { __builtin_amdgcn_s_setprio(2); dma_fill(U, tbn); __builtin_amdgcn_s_setprio(0); }\n";
             } else if (pi + 1 < g.npasses) {
@@ -923,7 +952,7 @@ inline std::string group_kernel(const CodegenOptions &opt, int gi, const GroupVi
              "    passes(T, U, more, tn, prev, tprev);\n";
         o += "    dma_wait();\n"
              "    __syncthreads();\n"
-             "    if (!more) { __builtin_amdgcn_s_setprio(2); store(T, tb); break; }\n"
+             "    if (!more) { __builtin_amdgcn_s_setprio(2); store_pairs(T, tb); break; }\n"
              "    prev = true;\n    tprev = tb;\n"
              "    c ^= 1;\n    ti = tn;\n    tb = tile_base(ti);\n"
              "  }\n";
