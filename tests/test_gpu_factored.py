@@ -89,11 +89,12 @@ def test_misfit_classes_run_unfactored(cases, graphs):
 
 @pytest.mark.parametrize('factor', [False, True])
 def test_ring_pipeline_equals_direct_store(factor):
-    """The ring of compact input buffers (lds_buffers=5: the live inputs of the next two tiles in flight,
-    waits counted per wave) runs the same arithmetic as the direct-store pipeline (lds_buffers=4): on
-    the 32q/d40/120T ring plan (width 29, 4 GiB complex64 frontier, six of its groups take the ring)
-    every block equals it bit for bit, per-step rounding and f0-factored, over four protocol prefixes
-    two lanes at a time."""
+    """The three asynchronous-load pipelines run the same arithmetic: the ring of compact input buffers
+    (lds_buffers=5: the live inputs of the next two tiles in flight, waits counted per wave), the
+    direct-store pipeline (4: the last pass stores one entry per lane from registers) and the default
+    (3: finished tiles stored from LDS two entries, 16 bytes, per lane).  On the 32q/d40/120T ring plan
+    (width 29, 4 GiB complex64 frontier, six of its groups take the ring) every block is the same bit
+    for bit, per-step rounding and f0-factored, over four protocol prefixes two lanes at a time."""
     from dcs_amd.sampling_protocol import make_protocol
     from dcs_amd.width_planner import best_rotation
     inst = synthetic_instance(32, 40, 120, seed=0)
@@ -103,7 +104,7 @@ def test_ring_pipeline_equals_direct_store(factor):
     kw = dict(dtype=np.complex64, schedule='compiled', lanes=2, blocks=2, group_policy=1, exact_rounding=not factor,
               factor=factor)
     out = {}
-    for buffers in (4, 5):
+    for buffers in (3, 4, 5):
         print(f'case 32q ring plan lds_buffers={buffers} factor={factor}', flush=True)
         e = InplaceConditionalBlockGPU(*args)
         e.alloc(lds_buffers=buffers, **kw)
@@ -112,5 +113,5 @@ def test_ring_pipeline_equals_direct_store(factor):
             vecs, _ = e.block_many(xs[i:i + 2])
             out[buffers] += [np.asarray(v) for v in vecs]
         e.close()
-    for a, b in zip(out[4], out[5]):
-        assert np.array_equal(a, b)
+    for a, b, c in zip(out[3], out[4], out[5]):
+        assert np.array_equal(a, b) and np.array_equal(b, c)
