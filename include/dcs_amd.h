@@ -119,7 +119,8 @@ typedef struct {
     int32_t lds_buffers;        /* DCS_MODE_COMPILED: LDS tiles per workgroup, 2 (double-buffered, the
                                    default when 0) or 1 (the next tile waits in registers); 3: two
                                    tiles, the next one loaded from HBM straight into LDS
-                                   (global_load_lds) during the current one's passes, in groups whose
+                                   (global_load_lds) during the current one's passes and finished
+                                   tiles stored two entries (16 bytes) per lane, in groups whose
                                    complex64 tiles allow it (else as 1); same results as 1;
                                    4: as 3, but the last register pass stores its results to HBM
                                    from registers and the group's scale is folded into its first
