@@ -20,6 +20,7 @@ from pathlib import Path
 import numpy as np
 
 DEFAULT_CANDIDATES = (
+    {'fuse_max_dim': 9, 'tile_log2': 12, 'group_policy': 3, 'lds_buffers': 3},
     {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 3, 'lds_buffers': 3},
     {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 3, 'lds_buffers': 4},
     {'fuse_max_dim': 8, 'tile_log2': 12, 'group_policy': 3, 'lds_buffers': 5},

@@ -264,9 +264,12 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    FP64 tiles for complex64, so twice the workgroups per CU); values are then rounded
                    to storage once per register pass
         fuse_max_dim, tile_log2, pass_max_dim  largest coset dimension of a fused group, log2 entries
-                   of its LDS tile (11, 12; 13 compiled only), span of one register pass (3, 4); the
-                   defaults are the fastest measured on the 64-qubit plans: (8, 12, 3), and (9, 13, 3)
-                   for complex128 in the compiled schedule when neither of the first two is given
+                   of its LDS tile (11, 12; 13 compiled only), span of one register pass (3, 4); in
+                   the compiled schedule a group of dimension 9 keeps rows of 16 entries on a tile of
+                   2^13 entries when tile_log2 is smaller.  The defaults are the fastest measured on
+                   the 64-qubit plans: (9, 12, 3) for the f0-factored complex64 kernels (groups of
+                   dimension 9 on 2^13 tiles, the others on 2^12), (9, 13, 3) for complex128 in the
+                   compiled schedule when neither of the first two is given, else (8, 12, 3)
         group_policy  0: greedy maximal coset groups; 1: the group boundaries with the fewest HBM
                    bytes (dynamic programming); 2: bytes and register-pass work summed; 3: the same two
                    costs overlapped, max plus a quarter of the smaller (both calibrated on complex64
@@ -304,12 +307,12 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
         if bp.terminal_rank > bp.k:
             raise ValueError(f'terminal rank {bp.terminal_rank} exceeds the {bp.k} open vertices')
         self.mode = SCHEDULES[schedule]
-        if fuse_max_dim is None and tile_log2 is None and self.mode == _native.MODE_COMPILED and self.use_fp64:
-            fuse_max_dim, tile_log2 = 9, 13
-        fuse_max_dim = 8 if fuse_max_dim is None else fuse_max_dim
-        tile_log2 = 12 if tile_log2 is None else tile_log2
         factored = (self.mode == _native.MODE_COMPILED and not self.use_fp64 and self._c64_storage == _native.STORAGE_C64
                     and lds_storage and not exact_rounding and not (factor is False or factor is None))
+        if fuse_max_dim is None and tile_log2 is None and self.mode == _native.MODE_COMPILED and self.use_fp64:
+            fuse_max_dim, tile_log2 = 9, 13
+        fuse_max_dim = (9 if factored else 8) if fuse_max_dim is None else fuse_max_dim
+        tile_log2 = 12 if tile_log2 is None else tile_log2
         if group_policy is None:
             group_policy = 3 if factored else 1
         if lds_buffers is None:

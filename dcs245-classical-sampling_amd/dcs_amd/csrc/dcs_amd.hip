@@ -1077,7 +1077,7 @@ bool step_is_resident(const StepHost &s, int rb) {
 // subspace of dimension <= D (a single step always qualifies), every slot grown inside the run is a
 // pivot of that subspace, and every slot dead at the run's start or end is one too (the coset
 // kernels zero-fill / skip exactly those coordinates).
-std::vector<int> group_ends_from(const dcs_plan *p, int s, int D) {
+std::vector<int> group_ends_from(const dcs_plan *p, int s, int D, std::vector<int> *dims = nullptr) {
     std::vector<int> ends;
     std::vector<u64> V;
     u64 Aun = p->steps[s].in_mask;
@@ -1088,7 +1088,10 @@ std::vector<int> group_ends_from(const dcs_plan *p, int s, int D) {
         u64 piv = 0;
         for (u64 v : V) piv |= 1ull << top_bit(v);
         const u64 dead = (Aun & ~p->steps[s].in_mask) | (Aun & ~p->steps[e].out_mask);
-        if (!(piv & ~Aun) && !(dead & ~piv)) ends.push_back(e + 1);
+        if (!(piv & ~Aun) && !(dead & ~piv)) {
+            ends.push_back(e + 1);
+            if (dims) dims->push_back((int)V.size());
+        }
     }
     return ends;
 }
@@ -1103,6 +1106,23 @@ CodegenOptions codegen_options(const dcs_plan *p, bool factored) {
 
 // Whether the plan's compiled schedule runs f0-factored group kernels by default.
 bool factoring_on(const dcs_plan *p) { return codegen_options(p, true).factored(); }
+
+// Tile size of a group of coset dimension d: 2^TL entries, except that a group wider than the generic
+// kernels take (only generated kernels run it) keeps rows of at least 16 entries (128 bytes in complex64)
+// with tiles of up to 2^13 entries when its LDS tiles hold 8 bytes an entry (complex64 or binary16
+// storage in LDS: two such tiles take 128 of the 160 KiB); 64-byte rows cost more than the larger tile
+// (dimension 9 on 2^12 tiles: 405 ms of group kernels per 64-qubit sample against 358 ms with 2^13
+// tiles for those groups).
+int group_tile_log2(const dcs_plan *p, int d) {
+    const int TL = p->tile_log2;
+    const bool lds8 = p->lds_storage != 0 && p->storage != DCS_STORAGE_C128;
+    return (d > kMaxDim && TL < 13 && lds8) ? std::max(TL, std::min(13, d + 4)) : TL;
+}
+
+// HBM cost factor of a group on a wider tile than the plan's (one 1024-thread workgroup per CU instead of
+// two of 512): dimension 9 on 2^13 tiles moved 6% fewer bytes in the same time as dimension 8 on 2^12
+// tiles on the 64-qubit plan; 1.05 and 1.1 both give 350 ms against 354 ms for 1.0.
+constexpr double kWideTileCost = 1.1;
 
 // Group boundaries over the non-resident runs of the plan.  Policy 0: greedy maximal groups (extend
 // while the span stays <= D).  Policy 1: the partition with the fewest HBM bytes, by dynamic
@@ -1159,9 +1179,13 @@ void choose_groups(const dcs_plan *p, int D, std::vector<int> &group_end) {
                 if (best[i - s] >= inf) continue;
                 u64 span = p->steps[i].in_mask;
                 int last = i;
-                for (int e : group_ends_from(p, i, D)) {
+                std::vector<int> dims;
+                const std::vector<int> ends = group_ends_from(p, i, D, &dims);
+                for (size_t k = 0; k < ends.size(); ++k) {
+                    const int e = ends[k];
                     for (; last < e; ++last) span |= p->steps[last].out_mask;
-                    const double tm = w_mem * (live(p->steps[i].in_mask) + live(p->steps[e - 1].out_mask));
+                    const double wm = group_tile_log2(p, dims[k]) > p->tile_log2 ? w_mem * kWideTileCost : w_mem;
+                    const double tm = wm * (live(p->steps[i].in_mask) + live(p->steps[e - 1].out_mask));
                     const double tc = w_cmp * (double)(e - i) * live(span);
                     const double c = best[i - s] + w_mem * std::ldexp(1.0, 20) +
                                      (overlap ? std::max(tm, tc) + 0.25 * std::min(tm, tc) : tm + tc);
@@ -1236,7 +1260,7 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
         }
         const u64 R = Aun & ~piv;
         const int rbits = popc(R);
-        int wb = std::min(TL - d, rbits);
+        int wb = std::min(group_tile_log2(p, d) - d, rbits);
         if (wb < 0) wb = 0;
         u64 low = 0;
         {

@@ -105,7 +105,9 @@ typedef struct {
     /* schedule knobs (DCS_MODE_FUSED) */
     int32_t fuse_max_dim;       /* largest coset dimension of a fused group: 1..8 (DCS_MODE_FUSED), 1..12 */
     int32_t resident_bits;      /* steps whose live slots all sit below this bit run LDS-resident, <= 12 */
-    int32_t tile_log2;          /* log2 entries per coset tile, 11 or 12 (13: DCS_MODE_COMPILED) */
+    int32_t tile_log2;          /* log2 entries per coset tile, 11 or 12 (13: DCS_MODE_COMPILED); in
+                                   DCS_MODE_COMPILED a group of dimension 9 takes 2^13-entry tiles when
+                                   this is smaller (rows of 16 entries), priced 1.1 per entry moved */
     int32_t exact_rounding;     /* 1: fused steps round to storage precision after every step (stored
                                    mantissas equal the per-step schedule); 0: round once per register
                                    pass (more accurate, fewer conversions) */

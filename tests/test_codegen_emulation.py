@@ -33,6 +33,7 @@ CASES = [
     (0, 12, 8, 1, 4, True, 1),    # complex64, asynchronous loads, last pass stores from registers
     (0, 12, 8, 3, 5, True, 2),    # complex64, ring of compact input buffers two tiles ahead where a group allows
     (0, 12, 8, 2, 4, True, 3),    # the same with three workgroups and bytes-and-passes groups
+    (0, 12, 9, 3, 3, True, 2),    # groups of dimension 9 on 2^13 tiles beside 2^12-entry ones
     (0, 11, 6, 0, 2, True, 3),    # complex64, two register-fed LDS tiles on 2^11 tiles, greedy groups
     (1, 12, 8, 1, 1, False, 2),   # complex128, one LDS tile
     (2, 12, 8, 2, 3, True, 2),    # binary16 storage (one-tile pipeline), bytes-and-passes groups
@@ -121,7 +122,7 @@ def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, p
 
 
 # f0-factored kernels: (storage, tile_log2, fuse_max_dim, group_policy, lds_buffers, gridcap, tolerance)
-FACTORED = [(0, 12, 8, 1, 3, 1, 2e-6), (0, 12, 8, 3, 3, 2, 2e-6), (0, 12, 8, 3, 4, 3, 2e-6), (0, 12, 8, 0, 5, 1, 2e-6), (0, 12, 8, 2, 1, 2, 2e-6),
+FACTORED = [(0, 12, 8, 1, 3, 1, 2e-6), (0, 12, 9, 3, 3, 2, 2e-6), (0, 12, 8, 3, 3, 2, 2e-6), (0, 12, 8, 3, 4, 3, 2e-6), (0, 12, 8, 0, 5, 1, 2e-6), (0, 12, 8, 2, 1, 2, 2e-6),
             (0, 11, 6, 0, 2, 3, 2e-6), (1, 12, 8, 1, 1, 2, 1e-12), (2, 12, 8, 2, 3, 2, 2e-2)]
 
 
