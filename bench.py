@@ -437,9 +437,12 @@ def main():
     if a.schedule == 'compiled' and tf.exists():
         rec = json.loads(tf.read_text())
         # counter readings belong to one schedule: use them only for the same launches and bytes
+        tiling = {k: fast['options'][k] for k in ('fuse_max_dim', 'tile_log2', 'pass_max_dim', 'group_policy',
+                                                   'lds_buffers')}
         if (rec.get('launches_per_sample') == float(launches[2]) and rec.get('lanes_per_launch', 1) == fast['lanes']
                 and rec.get('algorithmic_bytes_per_sample') == coset_bytes
-                and bool(rec.get('f0_factored', False)) == bool(factored)):
+                and bool(rec.get('f0_factored', False)) == bool(factored)
+                and rec.get('tiling', tiling) == tiling):
             traffic = rec.get('hbm_bytes_per_sample')
             valu = rec.get('valu_instructions_per_sample')
     # Three ceilings of the group kernels, each over their own time per sample: HBM bytes (algorithmic),

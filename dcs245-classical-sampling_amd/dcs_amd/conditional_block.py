@@ -248,7 +248,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
 
     def alloc(self, dtype=np.complex64, device=0, fp64_kernel_path=None, lanes=1, schedule='compiled',
               fuse_max_dim=None, resident_bits=12, tile_log2=None, blocks=1, exact_rounding=False,
-              pass_max_dim=3, lds_storage=True, group_policy=None, lds_buffers=None, graphs=False, context=None,
+              pass_max_dim=None, lds_storage=True, group_policy=None, lds_buffers=None, graphs=False, context=None,
               factor=True):
         """Compile the in-place plan and allocate the device frontier (kernel_opt/inplace_gpu.py:85-105).
 
@@ -267,8 +267,9 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
                    of its LDS tile (11, 12; 13 compiled only), span of one register pass (3, 4); in
                    the compiled schedule a group of dimension 9 keeps rows of 16 entries on a tile of
                    2^13 entries when tile_log2 is smaller.  The defaults are the fastest measured on
-                   the 64-qubit plans: (9, 12, 3) for the f0-factored complex64 kernels (groups of
-                   dimension 9 on 2^13 tiles, the others on 2^12), (9, 13, 3) for complex128 in the
+                   the 64-qubit plans: (9, 12, 4) for the f0-factored complex64 kernels (groups of
+                   dimension 9 on 2^13 tiles, the others on 2^12; a group kernel that spills registers
+                   with span-4 passes runs its span-3 version), (9, 13, 3) for complex128 in the
                    compiled schedule when neither of the first two is given, else (8, 12, 3)
         group_policy  0: greedy maximal coset groups; 1: the group boundaries with the fewest HBM
                    bytes (dynamic programming); 2: bytes and register-pass work summed; 3: the same two
@@ -313,6 +314,7 @@ class InplaceConditionalBlockGPU(ConditionalBlockGPU):
             fuse_max_dim, tile_log2 = 9, 13
         fuse_max_dim = (9 if factored else 8) if fuse_max_dim is None else fuse_max_dim
         tile_log2 = 12 if tile_log2 is None else tile_log2
+        pass_max_dim = (4 if factored else 3) if pass_max_dim is None else pass_max_dim
         if group_policy is None:
             group_policy = 3 if factored else 1
         if lds_buffers is None:

@@ -944,6 +944,7 @@ struct dcs_plan {
     std::vector<hipFunction_t> group_fn;
     std::vector<unsigned> group_grid;
     std::vector<int> group_threads;
+    std::vector<int> group_pass_span;  // register-pass span each compiled group kernel was built with
     // f0 factoring of the group kernels: per-step coefficient classes (empty: off); when on, group_fn holds
     // the factored kernels and group_fn_plain the unfactored ones, compiled when a contraction's
     // parameters first do not fit the classes; call_factored says which set the staged call runs
@@ -1209,9 +1210,12 @@ void choose_groups(const dcs_plan *p, int D, std::vector<int> &group_end) {
     }
 }
 
+// pass_span > 0 forms the register passes with that span instead of the plan's pass_max_dim (the
+// schedules and groups do not depend on it).
 int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<PassDev> &passes,
-                    std::vector<PStepDev> &psteps, std::vector<u64> &addrv) {
+                    std::vector<PStepDev> &psteps, std::vector<u64> &addrv, int pass_span = 0) {
     const int n = p->nsteps;
+    const int JP = pass_span > 0 ? pass_span : p->pass_max_dim;
     p->sched_steps.clear();
     for (int s = 0; s < n; ++s) p->sched_steps.push_back(Seg{0, s, 1, 0, -1, s});
     p->units_steps = n;
@@ -1383,7 +1387,7 @@ int build_schedules(dcs_plan *p, std::vector<GroupDev> &groups, std::vector<Pass
         auto express = [&](PassState &P, uint32_t m, int r, uint32_t &c) -> bool {
             const uint32_t rem = reduce(P, m, c);
             if (!rem) return P.is_pivot(r);
-            if ((int)P.basis.size() >= p->pass_max_dim) return false;
+            if ((int)P.basis.size() >= JP) return false;
             int piv;
             if ((rem >> r) & 1) piv = r;
             else if (P.is_pivot(r)) piv = top32(rem);
@@ -1579,13 +1583,10 @@ std::string group_module_source(const dcs_plan *p, const std::vector<GroupDev> &
     return src;
 }
 
-// Compile (or find in the module cache) the plan's group kernels, factored or not, and size their
-// persistent grids from their occupancy.
-int compile_module(dcs_plan *p, bool factored, std::vector<hipFunction_t> &fn, std::vector<unsigned> &grid,
-                   double *seconds) {
-    const std::string src = group_module_source(p, p->h_groups, p->h_passes, p->h_psteps, p->h_addrv, factored);
+// Compile a module source for the plan's device with hipRTC, or find it in the module cache.
+int load_module(dcs_plan *p, const std::string &src, hipModule_t &mod, double *seconds) {
     HIPCHK(hipSetDevice(p->ctx->device));
-    hipModule_t mod = nullptr;
+    mod = nullptr;
     {
         ModuleCache &mc = module_cache();
         std::lock_guard<std::mutex> lk(mc.mu);
@@ -1620,12 +1621,52 @@ int compile_module(dcs_plan *p, bool factored, std::vector<hipFunction_t> &fn, s
             if (seconds) *seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         }
     }
+    return DCS_OK;
+}
+
+// Compile (or find in the module cache) the plan's group kernels, factored or not, and size their
+// persistent grids from their occupancy.  With register passes of span 4, a kernel that spills
+// registers to scratch memory is replaced by the same group's kernel with passes of span 3 (a second
+// module; the groups, their tiles and their launches are the same): on the 64-qubit plan span 4 is 1.8%
+// faster over all, but a spilling kernel ran up to 27% slower than its span-3 version.
+int compile_module(dcs_plan *p, bool factored, std::vector<hipFunction_t> &fn, std::vector<unsigned> &grid,
+                   double *seconds) {
+    const std::string src = group_module_source(p, p->h_groups, p->h_passes, p->h_psteps, p->h_addrv, factored);
+    hipModule_t mod = nullptr;
+    {
+        const int rc = load_module(p, src, mod, seconds);
+        if (rc) return rc;
+    }
     fn.assign(p->h_groups.size(), nullptr);
     grid.assign(p->h_groups.size(), 1);
     p->group_threads.assign(p->h_groups.size(), kThreads);
+    std::vector<size_t> spilled;
     for (size_t gi = 0; gi < p->h_groups.size(); ++gi) {
         const std::string name = "dcs_group_" + std::to_string(gi);
         HIPCHK(hipModuleGetFunction(&fn[gi], mod, name.c_str()));
+        int scratch = 0;
+        HIPCHK(hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, fn[gi]));
+        if (scratch > 0) spilled.push_back(gi);
+    }
+    p->group_pass_span.assign(p->h_groups.size(), p->pass_max_dim);
+    if (!spilled.empty() && p->pass_max_dim > 3) {
+        std::vector<GroupDev> groups;
+        std::vector<PassDev> passes;
+        std::vector<PStepDev> psteps;
+        std::vector<u64> addrv;
+        int rc = build_schedules(p, groups, passes, psteps, addrv, 3);
+        if (rc) return rc;
+        if (groups.size() != p->h_groups.size()) return fail(DCS_ERR_RUNTIME, "span-3 schedule has other groups");
+        hipModule_t mod3 = nullptr;
+        rc = load_module(p, group_module_source(p, groups, passes, psteps, addrv, factored), mod3, seconds);
+        if (rc) return rc;
+        for (size_t gi : spilled) {
+            const std::string name = "dcs_group_" + std::to_string(gi);
+            HIPCHK(hipModuleGetFunction(&fn[gi], mod3, name.c_str()));
+            p->group_pass_span[gi] = 3;
+        }
+    }
+    for (size_t gi = 0; gi < p->h_groups.size(); ++gi) {
         p->group_threads[gi] = codegen::group_threads(p->h_groups[gi]);
         int nb = 0;
         HIPCHK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn[gi], p->group_threads[gi], 0));

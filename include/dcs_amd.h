@@ -111,7 +111,9 @@ typedef struct {
     int32_t exact_rounding;     /* 1: fused steps round to storage precision after every step (stored
                                    mantissas equal the per-step schedule); 0: round once per register
                                    pass (more accurate, fewer conversions) */
-    int32_t pass_max_dim;       /* span of the orbit masks of one register pass, 3 or 4 */
+    int32_t pass_max_dim;       /* span of the orbit masks of one register pass, 3 or 4 (4 in
+                                   DCS_MODE_COMPILED: a group kernel that spills registers runs with
+                                   span 3) */
     int32_t lds_storage;        /* DCS_MODE_COMPILED: 1 holds coset tiles in LDS in storage precision
                                    (rounded once per register pass), 0 in FP64 */
     int32_t group_policy;       /* 0: greedy maximal coset groups; 1: the group boundaries with the fewest
@@ -179,7 +181,9 @@ int dcs_plan_factoring(dcs_plan *plan, int32_t *module_factored, int32_t *last_c
  * group segments of the compiled schedule, without a device.  src (capacity src_cap, may be NULL)
  * receives the NUL-terminated source, cut to fit; *src_len its full length.  For each group segment i
  * < cap: group[i] the kernel index (dcs_group_<index>), first[i] and count[i] its steps, threads[i]
- * its workgroup size; *ngroups the number of segments.  The output arrays may be NULL. */
+ * its workgroup size; *ngroups the number of segments.  The output arrays may be NULL.  With
+ * pass_max_dim 4, dcs_plan_compile also builds the span-3 kernel of every group whose span-4 kernel
+ * spills registers to scratch memory, and runs that one instead (known only after compiling). */
 int dcs_plan_kernel_source(const dcs_plan_desc *desc, char *src, uint64_t src_cap, uint64_t *src_len,
                            int32_t *group, int32_t *first, int32_t *count, int32_t *threads, int cap,
                            int *ngroups);

@@ -71,13 +71,14 @@ def w17():
     return BlockPlan(inst['graph'], inst['unitaries'], 24, inst['natural_order'], [], list(range(16, 24)))
 
 
-def _emulate(w20, tmp_path, storage, tl, dim, policy, buffers, exact, gridcap, classes=None, tol=0.0):
-    """Generate the plan's group kernels, run them in the CPU emulator and return its report lines."""
+def _emulate(w20, tmp_path, storage, tl, dim, policy, buffers, exact, gridcap, classes=None, tol=0.0, pj=3):
+    """Generate the plan's group kernels (register passes of span pj), run them in the CPU emulator and
+    return its report lines."""
     t = w20.step_table()
     extra = {} if classes is None else {'coef_class': classes}
     src, segs = _native.kernel_source(t, w20.inplace_maxslots, storage, w20.terminal_mask(), w20.k,
                                       fuse_max_dim=dim, resident_bits=0, tile_log2=tl, exact_rounding=exact,
-                                      pass_max_dim=3, lds_storage=True, group_policy=policy, lds_buffers=buffers,
+                                      pass_max_dim=pj, lds_storage=True, group_policy=policy, lds_buffers=buffers,
                                       **extra)
     assert segs and len(set(s[0] for s in segs)) == len(segs)
     if buffers >= 3 and storage == 0:
@@ -124,6 +125,17 @@ def test_generated_kernels_match_steps_on_cpu(w20, tmp_path, storage, tl, dim, p
 # f0-factored kernels: (storage, tile_log2, fuse_max_dim, group_policy, lds_buffers, gridcap, tolerance)
 FACTORED = [(0, 12, 8, 1, 3, 1, 2e-6), (0, 12, 9, 3, 3, 2, 2e-6), (0, 12, 8, 3, 3, 2, 2e-6), (0, 12, 8, 3, 4, 3, 2e-6), (0, 12, 8, 0, 5, 1, 2e-6), (0, 12, 8, 2, 1, 2, 2e-6),
             (0, 11, 6, 0, 2, 3, 2e-6), (1, 12, 8, 1, 1, 2, 1e-12), (2, 12, 8, 2, 3, 2, 2e-2)]
+
+
+@pytest.mark.skipif(shutil.which('g++') is None, reason='needs the host C++ compiler')
+@pytest.mark.parametrize('exact', [True, False])
+def test_span4_passes_match_steps_on_cpu(w17, tmp_path, exact):
+    """Register passes over up to four basis vectors (16-entry orbits, the f0-factored default) with
+    groups of dimension 9 on 2^13 tiles beside 2^12-entry ones, on the width-17 plan: per-step rounding
+    bit for bit, and f0-factored within 2e-6 of the step formulas."""
+    classes = None if exact else np.random.default_rng(5).integers(0, 4, len(w17.inplace_steps)).astype(np.int32)
+    r, lines = _emulate(w17, tmp_path, 0, 12, 9, 3, 3, exact, 2, classes=classes, tol=0.0 if exact else 2e-6, pj=4)
+    assert r.returncode == 0 and lines[-1] == 'RESULT ok (0 bad)', r.stdout[-2000:]
 
 
 @pytest.mark.skipif(shutil.which('g++') is None, reason='needs the host C++ compiler')
